@@ -1,0 +1,167 @@
+"""Benchmark of the kernel-convolution dose path on one MI355X.
+
+Default workload (the CT-sized single volume): 256x256x200 FP32 activity at
+2.0 mm, liver ellipsoid + 3 lesions, density map with lung and bone, Lu-177
+31^3 dose-point kernel, reference (circular) convolution semantics, inputs
+resident in HBM.  A step is one full dose-rate map (five HIP passes).
+
+Prints one JSON line with throughput (Gvoxel/s), the roofline of the dominant
+kernel (HIP-event timed), the error against the FP64 oracle and the CPU
+baseline (the restated reference expression, np.fft complex128, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gvoxel/s dose conv 256x256x200 (FP64 err≤1e-5); CNN-corrected ms/vol @ ≤2% err"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fft_x_spectrum_multiply_ct_256x256x200.md")
+
+
+def pass_bytes(plan, T: int, density: bool):
+    """Algorithmic (compulsory) bytes of each FFT pass: its own inputs and outputs.
+
+    FZ reads T FP32 maps, writes the half spectrum Nx*Ny*H complex64; FY reads
+    that and writes Nx*Py*H; X reads Nx*Py*H plus the kernel spectrum
+    Px*Py*H and writes Nx*Py*H; IY reads Nx*Py*H, writes Nx*Ny*H; IZ reads
+    Nx*Ny*H (+ FP32 density) and writes the FP32 dose.
+    """
+    nx, ny, nz = plan.shape
+    px, py, pz = plan.fft_n
+    h = pz // 2 + 1
+    n = nx * ny * nz
+    c = 8
+    return {
+        "fft_z_forward": 4 * T * n + c * nx * ny * h,
+        "fft_y_forward": c * nx * ny * h + c * nx * py * h,
+        "fft_x_spectrum_multiply": c * nx * py * h * 2 + c * px * py * h,
+        "fft_y_inverse": c * nx * py * h + c * nx * ny * h,
+        "fft_z_inverse_epilogue": c * nx * ny * h + 4 * n * (2 if density else 1),
+    }
+
+
+def read_traffic(kernel_name: str):
+    """HBM bytes per launch of `kernel_name` from the committed rocprof summary, if present."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", text)
+    if m and kernel_name in text:
+        return float(m.group(1))
+    return None
+
+
+def cpu_baseline(act64, kernel64, reps: int):
+    """The reference expression (kernel_convolution.py:71-74) restated, timed on the host."""
+    from oracle import conv
+    n = act64.size
+    conv.ref_conv(act64, kernel64)                       # warm-up
+    ts = []
+    d64 = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d64 = conv.ref_conv(act64, kernel64)
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    med = ts[len(ts) // 2]
+    return {"value": n / med / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} full 256x256x200 calls of np.fft.ifftn(fftn(A)*fftn(K,s)).real "
+                      f"(complex128, numpy unthreaded) after 1 warm-up; median {med:.3f} s"}, d64
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 error check")
+    ap.add_argument("--profile-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    from pyvoxeldosimetry_amd.phantoms import ct_phantom
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ph = ct_phantom(seed=2)
+    kern = Lu177KernelGenerator("water").generate_kernel_device(ph.voxel_mm, ph.kernel_grid, device=dev)
+    act = torch.from_numpy(ph.activity[0]).to(dev)
+    rho = torch.from_numpy(ph.density).to(dev)
+    out = torch.empty_like(act)
+    plan = ConvPlan(ph.shape, ph.kernel_grid, mode="reference", algo="auto", device=dev)
+    plan.set_kernel(kern)
+    nvox = act.numel()
+
+    for _ in range(args.warmup):
+        plan.dose(act, density=rho, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.dose(act, density=rho, out=out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt * 1e3 / args.steps
+    gvox = nvox / (ms_step / 1e3) / 1e9
+
+    # per-kernel durations: HIP events recorded between the passes on the launch stream
+    prof = ConvPlan(ph.shape, ph.kernel_grid, mode="reference", algo=plan.algo,
+                    fft_n=plan.fft_n if plan.algo == "fft" else None, profile=True, device=dev)
+    prof.set_kernel(kern)
+    acc = {}
+    for i in range(args.profile_reps + 3):
+        prof.dose(act, density=rho, out=out)
+        t = prof.timings()
+        if i >= 3:
+            for k, v in t.items():
+                acc[k] = acc.get(k, 0.0) + v / args.profile_reps
+    dom = max(acc, key=acc.get)
+    roof = None
+    if plan.algo == "fft":
+        pb = pass_bytes(plan, 1, True)
+        achieved = pb[dom] / (acc[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom), "kernel": dom,
+                "kernel_ms": round(acc[dom], 5), "algorithmic_bytes": pb[dom]}
+    b_alg = nvox * (4 + 4 + 4) + 4 * int(np.prod(ph.kernel_grid))
+    operator_roof = {"algorithmic_bytes": b_alg, "achieved_GBs": round(b_alg / (ms_step * 1e-3) / 1e9, 1),
+                     "frac": round(b_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    result = {
+        "metric": METRIC, "value": round(gvox, 3), "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded CT phantom, seed 2)",
+        "config": {"workload": "ct single volume 256x256x200 f32 @2.0 mm, Lu-177 31^3 kernel, density map, "
+                               "reference (circular) convolution", "algo": plan.algo,
+                   "fft_n": list(plan.fft_n), "mode": "reference"},
+        "roofline": roof, "operator_roofline": operator_roof,
+        "per_pass_ms": {k: round(v, 5) for k, v in acc.items()},
+    }
+    if not args.no_cpu:
+        cb, d64 = cpu_baseline(ph.activity[0].astype(np.float64), kern.cpu().numpy(), args.cpu_reps)
+        result["cpu_baseline"] = cb
+        from oracle import conv
+        got = out.cpu().numpy()
+        m = conv.error_metrics(got, conv.epilogue(d64, ph.density))
+        result["error_vs_fp64"] = {"max_rel": m["max_rel"], "rms_rel": m["rms_rel"]}
+        result["speedup_vs_cpu_reference"] = round(gvox / cb["value"], 1)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

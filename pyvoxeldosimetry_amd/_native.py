@@ -1,0 +1,128 @@
+"""ctypes binding of ``libpvd.so`` (the C ABI declared in ``include/pvd.h``).
+
+The product path has no CPU fallback: if the library is missing or no HIP
+device is present, every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpvd.so")
+
+PVD_OK = 0
+PVD_EINVAL = -1
+PVD_ENOMEM = -2
+PVD_EHIP = -3
+PVD_ENOTSUP = -4
+
+PVD_MODE_REFERENCE = 0
+PVD_MODE_LINEAR = 1
+PVD_ALGO_AUTO = 0
+PVD_ALGO_FFT = 1
+PVD_ALGO_DIRECT = 2
+PVD_FLAG_PROFILE = 1
+PVD_MAX_TIMEPOINTS = 16
+PVD_MAX_FFT_LEN = 2048
+
+PVD_DPK_BETA = 0
+PVD_DPK_PHOTON = 1
+PVD_DPK_NAN_CENTRE = 1
+
+MODES = {"reference": PVD_MODE_REFERENCE, "linear": PVD_MODE_LINEAR}
+ALGOS = {"auto": PVD_ALGO_AUTO, "fft": PVD_ALGO_FFT, "direct": PVD_ALGO_DIRECT}
+
+
+class PlanParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("k", ctypes.c_int * 3), ("mode", ctypes.c_int),
+                ("algo", ctypes.c_int), ("fft_n", ctypes.c_int * 3), ("flags", ctypes.c_int)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("algo", ctypes.c_int), ("fft_n", ctypes.c_int * 3), ("kernel_used", ctypes.c_int * 3),
+                ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+
+
+class DpkTerm(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("flags", ctypes.c_int), ("amp", ctypes.c_double),
+                ("param", ctypes.c_double)]
+
+
+# every symbol include/pvd.h declares, with (restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+SIGNATURES = {
+    "pvd_last_error": (ctypes.c_char_p, []),
+    "pvd_version": (_I, []),
+    "pvd_dpk_generate": (_I, [_D, _I, _I, _I, ctypes.POINTER(DpkTerm), _I, _D, _P, _P]),
+    "pvd_plan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(PlanParams)]),
+    "pvd_plan_info_get": (_I, [_P, ctypes.POINTER(PlanInfo)]),
+    "pvd_plan_destroy": (_I, [_P]),
+    "pvd_plan_set_kernel": (_I, [_P, _P, _P]),
+    "pvd_conv_dose": (_I, [_P, _P, _P, _P, _D, _P]),
+    "pvd_conv_dose_weighted": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D), _I, _P, _P, _D, _P]),
+    "pvd_conv_dose_timeseries": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D), _I, _P, _P, _D, _P]),
+    "pvd_weighted_sum": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_D), _I, ctypes.c_size_t, _P, _P]),
+    "pvd_plan_timings": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "pvd_pass_name": (ctypes.c_char_p, [_P, _I]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def load(path: str | None = None):
+    """Load and type the library (does not touch the GPU)."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise NativeLibraryMissing(
+                f"{p} is missing: build it with `python -m pyvoxeldosimetry_amd.build` "
+                "(there is no CPU fallback)")
+        # torch first, so the HIP runtime it ships is the one libpvd binds to
+        import torch  # noqa: F401
+        lib = ctypes.CDLL(p)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def check(rc: int, what: str = "pvd call"):
+    """Map a PVD status code to the reference's Python exception types."""
+    if rc >= 0:
+        return rc
+    msg = (load().pvd_last_error() or b"").decode(errors="replace")
+    text = f"{what}: {msg}"
+    if rc == PVD_EINVAL:
+        raise ValueError(text)
+    if rc == PVD_ENOMEM:
+        raise MemoryError(text)
+    if rc == PVD_ENOTSUP:
+        raise NotImplementedError(text)
+    raise RuntimeError(text)
+
+
+def require_device():
+    """The product path runs only on a HIP device."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyvoxeldosimetry_amd needs a ROCm GPU (HIP device); none is available")
+
+
+def current_stream_handle(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

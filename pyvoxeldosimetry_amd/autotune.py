@@ -1,0 +1,125 @@
+"""Measured choice of algorithm and FFT lengths per (grid, kernel, mode).
+
+Candidates: the FFT pipeline at the plan's default lengths and at a few other
+7-smooth lengths that do not alias, and the direct LDS convolution when the
+kernel extent allows.  Each is timed with HIP events on synthetic input; the
+winner is cached in a JSON file keyed by shape, kernel, mode and device name.
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+
+from .plan import ConvPlan
+
+_CACHE_PATH = os.environ.get("PVD_AUTOTUNE_CACHE",
+                             os.path.join(os.path.expanduser("~"), ".cache", "pyvoxeldosimetry_amd", "autotune.json"))
+_lock = threading.Lock()
+
+
+def _smooth(n: int) -> bool:
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def candidate_lengths(n: int, k: int, mode: str, count: int = 3):
+    """Non-aliasing 7-smooth FFT lengths for one axis, smallest first."""
+    if mode == "reference":
+        ku = min(k, n)
+        need = n + ku - 1
+        out = [n] if _smooth(n) else []
+    else:
+        c = k // 2
+        mlo, mhi = max(0, c - n + 1), min(k - 1, c + n - 1)
+        need = n + max(mhi - c, c - mlo)
+        out = []
+    m = need
+    while len(out) < count and m <= 2048:
+        if _smooth(m):
+            out.append(m)
+        m += 1
+    return out
+
+
+def _load_cache():
+    try:
+        with open(_CACHE_PATH) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(c):
+    try:
+        os.makedirs(os.path.dirname(_CACHE_PATH), exist_ok=True)
+        tmp = _CACHE_PATH + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(c, f, indent=1, sort_keys=True)
+        os.replace(tmp, _CACHE_PATH)
+    except OSError:
+        pass
+
+
+def time_plan(plan: ConvPlan, reps: int = 10, warmup: int = 3) -> float:
+    """Mean GPU ms per dose call on random input."""
+    import torch
+    a = torch.rand(plan.shape, dtype=torch.float32, device=plan.device)
+    out = torch.empty_like(a)
+    for _ in range(warmup):
+        plan.dose(a, out=out)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        plan.dose(a, out=out)
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6) -> ConvPlan:
+    import torch
+    dev = kernel_dev.device
+    key = json.dumps([list(shape), list(kernel_shape), mode, torch.cuda.get_device_name(dev)])
+    with _lock:
+        cache = _load_cache()
+        hit = cache.get(key)
+    if hit is not None:
+        plan = ConvPlan(shape, kernel_shape, mode=mode, algo=hit["algo"],
+                        fft_n=hit.get("fft_n") if hit["algo"] == "fft" else None, device=dev)
+        plan.set_kernel(kernel_dev)
+        return plan
+    cands = []
+    per_axis = [candidate_lengths(shape[i], kernel_shape[i], mode) for i in range(3)]
+    seen = set()
+    for i in range(max(len(a) for a in per_axis)):
+        fn = tuple(a[min(i, len(a) - 1)] for a in per_axis)
+        if fn not in seen:
+            seen.add(fn)
+            cands.append(("fft", fn))
+    cands = cands[:max_candidates]
+    if all(min(shape[i], kernel_shape[i]) <= 16 if mode == "reference" else kernel_shape[i] <= 16 for i in range(3)):
+        cands.append(("direct", None))
+    best = None
+    for algo, fn in cands:
+        try:
+            plan = ConvPlan(shape, kernel_shape, mode=mode, algo=algo, fft_n=fn, device=dev)
+        except (ValueError, NotImplementedError, MemoryError):
+            continue
+        plan.set_kernel(kernel_dev)
+        t = time_plan(plan)
+        if best is None or t < best[0]:
+            if best is not None:
+                best[1].close()
+            best = (t, plan, algo, fn)
+        else:
+            plan.close()
+    if best is None:
+        raise RuntimeError(f"no usable convolution plan for shape {shape}, kernel {kernel_shape}")
+    with _lock:
+        cache = _load_cache()
+        cache[key] = {"algo": best[2], "fft_n": list(best[3]) if best[3] else None, "ms": best[0]}
+        _save_cache(cache)
+    return best[1]

@@ -1,0 +1,531 @@
+// Host side of libpvd: the C ABI declared in include/pvd.h.
+//
+// Plans own their device buffers (twiddles, complex workspace, kernel
+// spectrum or reversed small kernel).  Dose calls only launch kernels on the
+// caller's stream; nothing in a dose call allocates or synchronises.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pvd_internal.h"
+
+// kernels (pvd_fft.hip, pvd_spectrum.hip, pvd_direct.hip)
+hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds, hipStream_t st);
+hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_batch_args &a, int grid, int block, size_t lds,
+                                     hipStream_t st);
+hipError_t pvd_set_fft_lds_limits(size_t bytes);
+hipError_t pvd_launch_dpk(double v, const int g[3], const pvd_dpk_term *terms, int nterms, double factor,
+                          double *out, hipStream_t st);
+hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
+                               const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st);
+size_t pvd_direct_lds_bytes(const int k[3]);
+hipError_t pvd_direct_set_lds_limit(size_t bytes);
+hipError_t pvd_launch_wsum(const float *const *maps, const float *w, int T, size_t n, float *out, hipStream_t st);
+hipError_t pvd_launch_direct(const int N[3], const int K[3], const int sh[3], int wrap, const float *const *acts,
+                             const float *w, int T, const float *kr, const float *dens, float *out, float scale,
+                             hipStream_t st);
+
+#define PVD_VERSION 100
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? PVD_ENOMEM : PVD_EHIP, "%s: %s (%d)", what, hipGetErrorString(e),
+                (int)e);
+}
+
+#define PVD_HIP(call, what)                         \
+    do {                                            \
+        hipError_t e_ = (call);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+constexpr size_t kFftLdsCap = 96 * 1024;   // per-workgroup LDS budget of the FFT passes
+
+bool is_smooth(int n) {
+    if (n < 1) return false;
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+int next_smooth(int n) {
+    if (n < 1) n = 1;
+    while (!is_smooth(n)) ++n;
+    return n;
+}
+
+int factor_radices(int n, unsigned char *radix) {
+    int ns = 0;
+    int p2 = 0;
+    while (n % 2 == 0) { n /= 2; ++p2; }
+    while (p2 >= 4) { radix[ns++] = 16; p2 -= 4; }
+    if (p2 == 3) radix[ns++] = 8;
+    else if (p2 == 2) radix[ns++] = 4;
+    else if (p2 == 1) radix[ns++] = 2;
+    for (int p : {3, 5, 7})
+        while (n % p == 0) { n /= p; radix[ns++] = (unsigned char)p; }
+    return n == 1 ? ns : -1;
+}
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+struct device_limits {
+    int dev = -1;
+    size_t max_lds = 64 * 1024;
+};
+
+std::mutex g_mu;
+std::map<int, bool> g_attr_done;
+
+int ensure_kernel_attrs(int dev, size_t *max_lds) {
+    int v = 0;
+    PVD_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "query LDS size");
+    *max_lds = (size_t)v;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_attr_done[dev]) {
+        PVD_HIP(pvd_set_fft_lds_limits(std::min(kFftLdsCap, (size_t)v)), "set FFT LDS limit");
+        PVD_HIP(pvd_direct_set_lds_limit((size_t)v), "set direct LDS limit");
+        g_attr_done[dev] = true;
+    }
+    return PVD_OK;
+}
+
+// per-process cache of FP32 twiddle tables built in FP64 (one per length and device)
+struct tw_key {
+    int dev, n;
+    bool operator<(const tw_key &o) const { return dev != o.dev ? dev < o.dev : n < o.n; }
+};
+std::map<tw_key, float2 *> g_tw;
+
+int get_twiddles(int dev, int n, const float2 **out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tw.find({dev, n});
+    if (it != g_tw.end()) { *out = it->second; return PVD_OK; }
+    std::vector<float2> h(n);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int m = 0; m < n; ++m) {
+        // exact reduction: angle = 2 pi m / n with m < n
+        const double a = two_pi * (double)m / (double)n;
+        h[m] = make_float2((float)cos(a), (float)(-sin(a)));
+    }
+    float2 *d = nullptr;
+    PVD_HIP(hipMalloc(&d, sizeof(float2) * n), "allocate twiddles");
+    PVD_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice), "upload twiddles");
+    g_tw[{dev, n}] = d;
+    *out = d;
+    return PVD_OK;
+}
+
+struct launch_cfg {
+    int grid = 0, block = 0;
+    size_t lds = 0;
+};
+
+}  // namespace
+
+struct pvd_plan {
+    pvd_plan_params prm{};
+    int dev = 0;
+    int algo = PVD_ALGO_FFT;
+    int N[3]{}, K[3]{}, Ku[3]{};   // grid, kernel, kernel extent used (crop)
+    int P[3]{};
+    int lo[3]{}, s[3]{};           // kernel placement (first used sample, shift)
+    bool has_kernel = false;
+    pvd_vol_d vol{};               // descriptor template
+    launch_cfg cfg[PVD_NPASS_FFT];
+    float2 *work = nullptr;
+    float2 *spec = nullptr;
+    float *kr = nullptr;           // direct path: reversed cropped kernel (FP32)
+    size_t bytes = 0;
+    // profiling
+    hipEvent_t ev[PVD_NPASS_FFT + 1]{};
+    int n_ev = 0;
+    bool profiled_call = false;
+};
+
+struct pvd_batch {
+    std::vector<pvd_plan *> plans;
+};
+
+namespace {
+
+int axis_setup(pvd_plan *p) {
+    const pvd_plan_params &q = p->prm;
+    const bool ref = q.mode == PVD_MODE_REFERENCE;
+    for (int a = 0; a < 3; ++a) {
+        const int N = q.n[a], K = q.k[a];
+        p->N[a] = N;
+        p->K[a] = K;
+        int need;          // smallest FFT length free of aliasing when padded
+        if (ref) {
+            const int Ku = std::min(K, N);           // np.fft.fftn(kernel, s=N) crops to [0:N]
+            p->Ku[a] = Ku;
+            p->lo[a] = 0;
+            p->s[a] = 0;
+            need = N + Ku - 1;
+        } else {
+            const int c = K / 2;
+            const int mlo = std::max(0, c - N + 1), mhi = std::min(K - 1, c + N - 1);
+            p->lo[a] = mlo;
+            p->Ku[a] = mhi - mlo + 1;
+            p->s[a] = c;
+            need = N + std::max(mhi - c, c - mlo);
+        }
+        int P = q.fft_n[a];
+        if (P == 0) P = (ref && is_smooth(N)) ? N : next_smooth(need);
+        if (!is_smooth(P)) return fail(PVD_EINVAL, "fft_n[%d]=%d is not 7-smooth", a, P);
+        if (ref ? !(P == N || P >= need) : P < need)
+            return fail(PVD_EINVAL, "fft_n[%d]=%d aliases: need %s%d", a, P, ref ? "N or >= " : ">= ", need);
+        if (P > PVD_MAX_FFT_LEN) return fail(PVD_ENOTSUP, "FFT length %d on axis %d exceeds %d", P, a,
+                                             PVD_MAX_FFT_LEN);
+        p->P[a] = P;
+        pvd_axis_d &ax = p->vol.ax[a];
+        ax.n = P;
+        ax.N = N;
+        ax.nst = factor_radices(P, ax.radix);
+        if (ax.nst < 0 || ax.nst > PVD_MAXSTAGE) return fail(PVD_EINVAL, "cannot factor FFT length %d", P);
+        // output o reads FFT index (o - s) mod P in reference mode, o in linear mode
+        ax.off = ref ? ((P - (p->s[a] % P)) % P) : 0;
+        ax.foldn = (ref && P != N) ? p->Ku[a] - 1 : 0;
+    }
+    return PVD_OK;
+}
+
+// lines per tile and block size so that lines*n <= 16*block and LDS fits
+void pick_tile(int n, int target, int extra_ld, int *lines, int *block, size_t *lds) {
+    const size_t cap_elems = kFftLdsCap / sizeof(float2);
+    int l = target;
+    const int twn = round_up(n, 8);
+    while (l > 1 && ((size_t)twn + (size_t)n * (l + extra_ld) > cap_elems || l * n > 16 * 1024)) --l;
+    *lines = l;
+    int b = round_up((l * n + 15) / 16, 64);
+    b = std::max(64, std::min(1024, b));
+    *block = b;
+    *lds = sizeof(float2) * ((size_t)twn + (size_t)n * (l + extra_ld));
+}
+
+int fft_setup(pvd_plan *p) {
+    pvd_vol_d &v = p->vol;
+    const int Nx = p->N[0], Ny = p->N[1];
+    const int Px = p->P[0], Py = p->P[1], Pz = p->P[2];
+    v.H = Pz / 2 + 1;
+    int lz, bz; size_t ldsz;
+    pick_tile(Pz, 16, 1, &lz, &bz, &ldsz);
+    int tky, by; size_t ldsy;
+    pick_tile(Py, std::min(16, v.H), 0, &tky, &by, &ldsy);
+    int tkx, bx; size_t ldsx;
+    pick_tile(Px, 16, 0, &tkx, &bx, &ldsx);
+    v.lz = lz;
+    v.tky = tky;
+    v.tkx = tkx;
+    const long long nlines = (long long)Nx * Ny;
+    const long long tz = (nlines + 2 * lz - 1) / (2 * lz);
+    const long long ty = (long long)Nx * ((v.H + tky - 1) / tky);
+    const long long plane = (long long)Py * v.H;
+    const long long tx = (plane + tkx - 1) / tkx;
+    if (tz > 0x7fffffffLL || ty > 0x7fffffffLL || tx > 0x7fffffffLL) return fail(PVD_ENOTSUP, "grid too large");
+    p->cfg[PVD_PASS_FZ] = {(int)tz, bz, ldsz};
+    p->cfg[PVD_PASS_IZ] = {(int)tz, bz, ldsz};
+    p->cfg[PVD_PASS_FY] = {(int)ty, by, ldsy};
+    p->cfg[PVD_PASS_IY] = {(int)ty, by, ldsy};
+    p->cfg[PVD_PASS_X] = {(int)tx, bx, ldsx};
+    for (int i = 0; i < PVD_NPASS_FFT; ++i) {
+        v.tiles[i] = p->cfg[i].grid;
+        v.tile_base[i] = 0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const float2 *tw = nullptr;
+        int rc = get_twiddles(p->dev, p->P[a], &tw);
+        if (rc) return rc;
+        v.ax[a].tw = tw;
+    }
+    const size_t work_elems = (size_t)Nx * Py * v.H;
+    const size_t spec_elems = (size_t)Px * Py * v.H;
+    PVD_HIP(hipMalloc(&p->work, sizeof(float2) * work_elems), "allocate FFT workspace");
+    PVD_HIP(hipMalloc(&p->spec, sizeof(float2) * spec_elems), "allocate kernel spectrum");
+    p->bytes = sizeof(float2) * (work_elems + spec_elems);
+    v.work = p->work;
+    v.spec = p->spec;
+    return PVD_OK;
+}
+
+double fft_cost_model(const pvd_plan *p) {
+    double vol = (double)p->N[0] * p->N[1] * p->N[2];
+    double padded = (double)p->N[0] * p->P[1] * (p->P[2] / 2 + 1) * 2.0;
+    return (vol * 12.0 + padded * 4.0 * 7.0) / 5.0e12 + 5 * 3.0e-6;
+}
+
+double direct_cost_model(const pvd_plan *p) {
+    double vol = (double)p->N[0] * p->N[1] * p->N[2];
+    return vol * 2.0 * p->Ku[0] * p->Ku[1] * p->Ku[2] / 40.0e12 + 3.0e-6;
+}
+
+int launch_fft(pvd_plan *p, const pvd_vol_d &v, hipStream_t st) {
+    const bool prof = (p->prm.flags & PVD_FLAG_PROFILE) != 0;
+    if (prof) PVD_HIP(hipEventRecord(p->ev[0], st), "record event");
+    for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
+        const launch_cfg &c = p->cfg[pass];
+        PVD_HIP(pvd_launch_fft_pass(pass, v, c.grid, c.block, c.lds, st), "launch FFT pass");
+        if (prof) PVD_HIP(hipEventRecord(p->ev[pass + 1], st), "record event");
+    }
+    p->profiled_call = prof;
+    return PVD_OK;
+}
+
+int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const float *dens, float *out,
+             double scale, hipStream_t st) {
+    if (!p) return fail(PVD_EINVAL, "null plan");
+    if (!p->has_kernel) return fail(PVD_EINVAL, "plan has no kernel: call pvd_plan_set_kernel first");
+    if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
+    if (!out) return fail(PVD_EINVAL, "null dose pointer");
+    for (int t = 0; t < T; ++t)
+        if (!acts[t]) return fail(PVD_EINVAL, "null activity pointer %d", t);
+    if (!isfinite(scale)) return fail(PVD_EINVAL, "non-finite scale");
+    PVD_HIP(hipSetDevice(p->dev), "set device");
+    if (p->algo == PVD_ALGO_DIRECT) {
+        const bool prof = (p->prm.flags & PVD_FLAG_PROFILE) != 0;
+        const int sh[3] = {p->prm.mode == PVD_MODE_LINEAR ? p->K[0] / 2 - p->lo[0] : 0,
+                           p->prm.mode == PVD_MODE_LINEAR ? p->K[1] / 2 - p->lo[1] : 0,
+                           p->prm.mode == PVD_MODE_LINEAR ? p->K[2] / 2 - p->lo[2] : 0};
+        if (prof) PVD_HIP(hipEventRecord(p->ev[0], st), "record event");
+        PVD_HIP(pvd_launch_direct(p->N, p->Ku, sh, p->prm.mode == PVD_MODE_REFERENCE, acts, w, T, p->kr, dens,
+                                  out, (float)scale, st),
+                "launch direct convolution");
+        if (prof) PVD_HIP(hipEventRecord(p->ev[1], st), "record event");
+        p->profiled_call = prof;
+        return PVD_OK;
+    }
+    pvd_vol_d v = p->vol;
+    v.T = T;
+    for (int t = 0; t < PVD_MAX_TIMEPOINTS; ++t) {
+        v.act[t] = t < T ? acts[t] : nullptr;
+        v.w[t] = t < T ? w[t] : 0.f;
+    }
+    v.dens = dens;
+    v.out = out;
+    v.scale = (float)scale;
+    return launch_fft(p, v, st);
+}
+
+}  // namespace
+
+// =================================================================================
+extern "C" {
+
+const char *pvd_last_error(void) { return g_err.c_str(); }
+
+int pvd_version(void) { return PVD_VERSION; }
+
+int pvd_dpk_generate(double voxel_mm, int gx, int gy, int gz, const pvd_dpk_term *terms, int nterms,
+                     double factor, double *out, void *stream) {
+    if (gx < 1 || gy < 1 || gz < 1) return fail(PVD_EINVAL, "kernel grid must be positive");
+    if (nterms < 0 || nterms > 16 || (nterms > 0 && !terms)) return fail(PVD_EINVAL, "bad term list");
+    if (!out) return fail(PVD_EINVAL, "null output");
+    for (int i = 0; i < nterms; ++i)
+        if (terms[i].kind != PVD_DPK_BETA && terms[i].kind != PVD_DPK_PHOTON)
+            return fail(PVD_EINVAL, "unknown term kind %d", terms[i].kind);
+    const int g[3] = {gx, gy, gz};
+    PVD_HIP(pvd_launch_dpk(voxel_mm, g, terms, nterms, factor, out, (hipStream_t)stream), "launch DPK generator");
+    return PVD_OK;
+}
+
+int pvd_plan_create(pvd_plan **out, const pvd_plan_params *q) {
+    if (!out || !q) return fail(PVD_EINVAL, "null argument");
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (q->n[a] < 1) return fail(PVD_EINVAL, "activity grid extent %d on axis %d", q->n[a], a);
+        if (q->k[a] < 1) return fail(PVD_EINVAL, "kernel grid extent %d on axis %d", q->k[a], a);
+        if (q->fft_n[a] < 0) return fail(PVD_EINVAL, "negative fft_n");
+    }
+    if (q->mode != PVD_MODE_REFERENCE && q->mode != PVD_MODE_LINEAR) return fail(PVD_EINVAL, "bad mode %d", q->mode);
+    if (q->algo < PVD_ALGO_AUTO || q->algo > PVD_ALGO_DIRECT) return fail(PVD_EINVAL, "bad algo %d", q->algo);
+    pvd_plan *p = new (std::nothrow) pvd_plan();
+    if (!p) return fail(PVD_ENOMEM, "host allocation");
+    p->prm = *q;
+    int rc = PVD_OK;
+    do {
+        if (hipGetDevice(&p->dev) != hipSuccess) { rc = fail(PVD_EHIP, "no HIP device"); break; }
+        size_t max_lds = 0;
+        if ((rc = ensure_kernel_attrs(p->dev, &max_lds))) break;
+        if ((rc = axis_setup(p))) break;
+        const bool direct_ok = p->Ku[0] <= 16 && p->Ku[1] <= 16 && p->Ku[2] <= 16 &&
+                               pvd_direct_lds_bytes(p->Ku) <= max_lds;
+        int algo = q->algo;
+        if (algo == PVD_ALGO_DIRECT && !direct_ok) {
+            rc = fail(PVD_ENOTSUP, "direct path needs kernel extents <= 16 (got %dx%dx%d)", p->Ku[0], p->Ku[1],
+                      p->Ku[2]);
+            break;
+        }
+        if (algo == PVD_ALGO_AUTO)
+            algo = (direct_ok && direct_cost_model(p) < fft_cost_model(p)) ? PVD_ALGO_DIRECT : PVD_ALGO_FFT;
+        p->algo = algo;
+        if (algo == PVD_ALGO_FFT) {
+            if ((rc = fft_setup(p))) break;
+        } else {
+            const size_t nk = (size_t)p->Ku[0] * p->Ku[1] * p->Ku[2];
+            if (hipMalloc(&p->kr, sizeof(float) * nk) != hipSuccess) { rc = fail(PVD_ENOMEM, "allocate kernel"); break; }
+            p->bytes = sizeof(float) * nk;
+        }
+        if (q->flags & PVD_FLAG_PROFILE) {
+            const int nev = (algo == PVD_ALGO_FFT) ? PVD_NPASS_FFT + 1 : 2;
+            for (int i = 0; i < nev; ++i)
+                if (hipEventCreate(&p->ev[i]) != hipSuccess) { rc = fail(PVD_EHIP, "create event"); break; }
+            p->n_ev = nev;
+        }
+    } while (0);
+    if (rc) {
+        pvd_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return PVD_OK;
+}
+
+int pvd_plan_info_get(const pvd_plan *p, pvd_plan_info *info) {
+    if (!p || !info) return fail(PVD_EINVAL, "null argument");
+    info->algo = p->algo;
+    for (int a = 0; a < 3; ++a) {
+        info->fft_n[a] = p->algo == PVD_ALGO_FFT ? p->P[a] : 0;
+        info->kernel_used[a] = p->Ku[a];
+    }
+    info->n_passes = p->algo == PVD_ALGO_FFT ? PVD_NPASS_FFT : 1;
+    info->workspace_bytes = p->bytes;
+    return PVD_OK;
+}
+
+int pvd_plan_destroy(pvd_plan *p) {
+    if (!p) return PVD_OK;
+    if (p->work) (void)hipFree(p->work);
+    if (p->spec) (void)hipFree(p->spec);
+    if (p->kr) (void)hipFree(p->kr);
+    for (int i = 0; i < p->n_ev; ++i)
+        if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+    delete p;
+    return PVD_OK;
+}
+
+int pvd_plan_set_kernel(pvd_plan *p, const double *kernel, void *stream) {
+    if (!p || !kernel) return fail(PVD_EINVAL, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    PVD_HIP(hipSetDevice(p->dev), "set device");
+    if (p->algo == PVD_ALGO_DIRECT) {
+        const size_t nk = (size_t)p->K[0] * p->K[1] * p->K[2];
+        std::vector<double> h(nk);
+        PVD_HIP(hipMemcpyAsync(h.data(), kernel, sizeof(double) * nk, hipMemcpyDeviceToHost, st), "download kernel");
+        PVD_HIP(hipStreamSynchronize(st), "synchronise");
+        const int *Ku = p->Ku, *lo = p->lo;
+        std::vector<float> r((size_t)Ku[0] * Ku[1] * Ku[2]);
+        for (int i = 0; i < Ku[0]; ++i)
+            for (int j = 0; j < Ku[1]; ++j)
+                for (int l = 0; l < Ku[2]; ++l) {
+                    const size_t src = ((size_t)(lo[0] + Ku[0] - 1 - i) * p->K[1] + (lo[1] + Ku[1] - 1 - j)) * p->K[2] +
+                                       (lo[2] + Ku[2] - 1 - l);
+                    r[((size_t)i * Ku[1] + j) * Ku[2] + l] = (float)h[src];
+                }
+        PVD_HIP(hipMemcpyAsync(p->kr, r.data(), sizeof(float) * r.size(), hipMemcpyHostToDevice, st), "upload kernel");
+        PVD_HIP(hipStreamSynchronize(st), "synchronise");
+        p->has_kernel = true;
+        return PVD_OK;
+    }
+    const int H = p->P[2] / 2 + 1;
+    double2 *T1 = nullptr, *T2 = nullptr;
+    const size_t n1 = (size_t)p->Ku[0] * p->Ku[1] * H, n2 = (size_t)p->Ku[0] * p->P[1] * H;
+    PVD_HIP(hipMalloc(&T1, sizeof(double2) * n1), "allocate spectrum scratch");
+    hipError_t e = hipMalloc(&T2, sizeof(double2) * n2);
+    if (e != hipSuccess) { (void)hipFree(T1); return hip_fail(e, "allocate spectrum scratch"); }
+    e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, kernel, T1, T2, p->spec, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(T1);
+    (void)hipFree(T2);
+    if (e != hipSuccess) return hip_fail(e, "kernel spectrum");
+    p->has_kernel = true;
+    return PVD_OK;
+}
+
+int pvd_conv_dose(pvd_plan *p, const float *act, const float *density, float *dose, double scale, void *stream) {
+    const float *acts[1] = {act};
+    const float w[1] = {1.0f};
+    return run_dose(p, acts, w, 1, density, dose, scale, (hipStream_t)stream);
+}
+
+int pvd_conv_dose_weighted(pvd_plan *p, const float *const *acts, const double *weights, int T,
+                           const float *density, float *dose, double scale, void *stream) {
+    if (!acts || !weights) return fail(PVD_EINVAL, "null argument");
+    if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
+    float w[PVD_MAX_TIMEPOINTS];
+    for (int t = 0; t < T; ++t) {
+        if (!isfinite(weights[t])) return fail(PVD_EINVAL, "non-finite weight %d", t);
+        w[t] = (float)weights[t];
+    }
+    return run_dose(p, acts, w, T, density, dose, scale, (hipStream_t)stream);
+}
+
+int pvd_conv_dose_timeseries(pvd_plan *p, const float *const *acts, const double *t_hours, int T,
+                             const float *density, float *dose, double scale, void *stream) {
+    if (!acts || !t_hours) return fail(PVD_EINVAL, "null argument");
+    if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
+    // trapezoid weights, kernel_convolution.py:101-104 (dt in hours * 3600)
+    double w[PVD_MAX_TIMEPOINTS] = {0};
+    for (int i = 0; i + 1 < T; ++i) {
+        const double dt = (t_hours[i + 1] - t_hours[i]) * 3600.0;
+        w[i] += dt / 2.0;
+        w[i + 1] += dt / 2.0;
+    }
+    return pvd_conv_dose_weighted(p, acts, w, T, density, dose, scale, stream);
+}
+
+int pvd_weighted_sum(const float *const *maps, const double *weights, int T, size_t n, float *out,
+                     void *stream) {
+    if (!maps || !weights || !out) return fail(PVD_EINVAL, "null argument");
+    if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
+    float w[PVD_MAX_TIMEPOINTS];
+    for (int t = 0; t < T; ++t) {
+        if (!maps[t]) return fail(PVD_EINVAL, "null map %d", t);
+        if (!isfinite(weights[t])) return fail(PVD_EINVAL, "non-finite weight %d", t);
+        w[t] = (float)weights[t];
+    }
+    if (n == 0) return PVD_OK;
+    PVD_HIP(pvd_launch_wsum(maps, w, T, n, out, (hipStream_t)stream), "launch weighted sum");
+    return PVD_OK;
+}
+
+int pvd_plan_timings(pvd_plan *p, float *ms, int n) {
+    if (!p) return fail(PVD_EINVAL, "null plan");
+    if (!p->profiled_call) return fail(PVD_EINVAL, "no profiled call recorded (create the plan with PVD_FLAG_PROFILE)");
+    const int np = p->n_ev - 1;
+    PVD_HIP(hipEventSynchronize(p->ev[np]), "synchronise event");
+    for (int i = 0; i < np && i < n; ++i)
+        PVD_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]), "event time");
+    return np;
+}
+
+const char *pvd_pass_name(const pvd_plan *p, int i) {
+    static const char *fft[PVD_NPASS_FFT] = {"fft_z_forward", "fft_y_forward", "fft_x_spectrum_multiply",
+                                             "fft_y_inverse", "fft_z_inverse_epilogue"};
+    if (!p) return "";
+    if (p->algo == PVD_ALGO_DIRECT) return i == 0 ? "direct_conv" : "";
+    return (i >= 0 && i < PVD_NPASS_FFT) ? fft[i] : "";
+}
+
+}  // extern "C"

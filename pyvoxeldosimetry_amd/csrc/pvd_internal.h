@@ -1,0 +1,275 @@
+// Internal descriptors and device helpers shared by the HIP sources of libpvd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pvd.h"
+
+#define PVD_MAXSTAGE 16
+
+// One FFT axis of one volume.
+//   n     FFT length P (7-smooth)
+//   N     logical length (activity grid extent)
+//   off   output o reads FFT index (o + off) mod n
+//   foldn outputs o < foldn also add FFT index (o + N + off) mod n
+//         (reference-mode circular wrap when the FFT is padded, P != N)
+struct pvd_axis_d {
+    int n;
+    int N;
+    int nst;
+    int off;
+    int foldn;
+    int pad_;
+    const float2 *tw;                 // n forward twiddles W_n^m, FP64-built, FP32-stored
+    unsigned char radix[PVD_MAXSTAGE];
+};
+
+enum { PVD_PASS_FZ = 0, PVD_PASS_FY = 1, PVD_PASS_X = 2, PVD_PASS_IY = 3, PVD_PASS_IZ = 4,
+       PVD_NPASS_FFT = 5 };
+
+// Everything one FFT dose call needs (passed by value as a kernel argument for a
+// single volume, or read from a device table for a batch).
+struct pvd_vol_d {
+    pvd_axis_d ax[3];                 // x, y, z
+    int H;                            // Pz/2 + 1 (half spectrum along z)
+    int T;                            // timepoints folded into the load
+    int lz;                           // complex line pairs per z-pass tile
+    int tky;                          // columns per y-pass tile
+    int tkx;                          // columns per x-pass tile
+    int pad_;
+    const float *act[PVD_MAX_TIMEPOINTS];
+    float w[PVD_MAX_TIMEPOINTS];
+    const float *dens;
+    float *out;
+    float scale;
+    int pad2_;
+    float2 *work;                     // [Nx][Py][H] complex workspace
+    const float2 *spec;               // [Px][Py][H] kernel spectrum (1/P folded in)
+    int tile_base[PVD_NPASS_FFT];     // first grid tile of this volume per pass (batch)
+    int tiles[PVD_NPASS_FFT];         // tiles of this volume per pass
+};
+
+struct pvd_batch_args {
+    const pvd_vol_d *vols;
+    int nvol;
+};
+
+// ---------------------------------------------------------------------------------
+// complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------------
+// small DFTs, forward: X[k] = sum_n x[n] e^{-2 pi i nk/R}; inverse: e^{+...}
+template <int R, bool INV> struct Dft;
+
+template <bool INV> struct Dft<2, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <bool INV> struct Dft<4, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    }
+};
+
+template <bool INV> struct Dft<8, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        const float r = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV>::run(e);
+        Dft<4, INV>::run(o);
+        // twiddles W8^k (forward e^{-i pi k/4}); inverse conjugated
+        float2 o1 = INV ? make_float2(r * (o[1].x - o[1].y), r * (o[1].x + o[1].y))
+                        : make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+        float2 o2 = mul_mi<INV>(o[2]);
+        float2 o3 = INV ? make_float2(-r * (o[3].x + o[3].y), r * (o[3].x - o[3].y))
+                        : make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+        v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+    }
+};
+
+template <bool INV> struct Dft<16, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 e[8], o[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+        Dft<8, INV>::run(e);
+        Dft<8, INV>::run(o);
+        // W16^k = cos(pi k/8) -/+ i sin(pi k/8)
+        const float c[8] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f,
+                            0.38268343236508977173f, 0.0f, -0.38268343236508977173f,
+                            -0.70710678118654752440f, -0.92387953251128675613f};
+        const float s[8] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f,
+                            0.92387953251128675613f, 1.0f, 0.92387953251128675613f,
+                            0.70710678118654752440f, 0.38268343236508977173f};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float2 w = make_float2(c[k], INV ? s[k] : -s[k]);
+            float2 t = (k == 0) ? o[0] : cmul(o[k], w);
+            v[k] = cadd(e[k], t);
+            v[k + 8] = csub(e[k], t);
+        }
+    }
+};
+
+template <bool INV> struct Dft<3, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        const float s3 = 0.86602540378443864676f;
+        float2 t = cadd(v[1], v[2]);
+        float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        float2 d = cscale(mul_mi<INV>(csub(v[1], v[2])), s3);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, d);
+        v[2] = csub(m, d);
+    }
+};
+
+template <bool INV> struct Dft<5, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        float2 b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+        float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+        float2 a0 = v[0];
+        float2 m1 = make_float2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+        float2 m2 = make_float2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+        float2 n1 = mul_mi<INV>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+        float2 n2 = mul_mi<INV>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+        v[0] = make_float2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+        v[1] = cadd(m1, n1);
+        v[4] = csub(m1, n1);
+        v[2] = cadd(m2, n2);
+        v[3] = csub(m2, n2);
+    }
+};
+
+template <bool INV> struct Dft<7, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        // cos/sin(2 pi j/7), j = 1..3
+        const float c[3] = {0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
+        const float s[3] = {0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f};
+        float2 b[3], d[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) { b[j] = cadd(v[j + 1], v[6 - j]); d[j] = csub(v[j + 1], v[6 - j]); }
+        float2 a0 = v[0];
+        float2 X[7];
+        X[0] = make_float2(a0.x + b[0].x + b[1].x + b[2].x, a0.y + b[0].y + b[1].y + b[2].y);
+#pragma unroll
+        for (int k = 1; k <= 3; ++k) {
+            float2 m = a0, n = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int q = ((j + 1) * k) % 7;           // angle index
+                const float cc = (q <= 3) ? c[q - 1] : c[7 - q - 1];
+                const float ss = (q <= 3) ? s[q - 1] : -s[7 - q - 1];
+                m.x += cc * b[j].x; m.y += cc * b[j].y;
+                n.x += ss * d[j].x; n.y += ss * d[j].y;
+            }
+            float2 t = mul_mi<INV>(n);
+            X[k] = cadd(m, t);
+            X[7 - k] = csub(m, t);
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) v[k] = X[k];
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// One Stockham stage of radix R over `nl` lines held in LDS, element
+// (line l, index k) at s[k*ld + l].  Ns = product of the radices of the stages
+// already done.  Every thread keeps up to IT butterflies in registers between
+// the read and the write phase so the stage is in place; the launch guarantees
+// nl*n <= 16*blockDim, hence nl*n/R <= IT*blockDim.
+template <int R> struct StageItems { static constexpr int v = (16 + R - 1) / R; };
+
+template <int R, bool INV>
+__device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int nl, int ld, int n, int Ns,
+                                               const float2 *__restrict__ tw) {
+    constexpr int IT = StageItems<R>::v;
+    const int m = n / R;
+    const int work = nl * m;
+    const int tstride = n / (Ns * R);
+    float2 v[IT][R];
+    int dst[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = threadIdx.x + it * blockDim.x;
+        dst[it] = -1;
+        if (w < work) {
+            const int j = w / nl;
+            const int l = w - j * nl;
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = s[(j + r * m) * ld + l];
+            if (k != 0) {
+                const int t = k * tstride;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 wv = tw[r * t];
+                    if (INV) wv.y = -wv.y;
+                    v[it][r] = cmul(v[it][r], wv);
+                }
+            }
+            Dft<R, INV>::run(v[it]);
+            dst[it] = ((j - k) * R + k) * ld + l;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        if (dst[it] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[dst[it] + r * Ns * ld] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full FFT of length ax.n over nl lines in LDS (natural order in and out,
+// unnormalised).  Must be entered after a barrier that published `s` and `tw`.
+template <bool INV>
+__device__ __forceinline__ void fft_lines(float2 *__restrict__ s, int nl, int ld, const pvd_axis_d &ax,
+                                          const float2 *__restrict__ tw) {
+    const int n = ax.n;
+    int Ns = 1;
+    for (int st = 0; st < ax.nst; ++st) {
+        const int R = ax.radix[st];
+        switch (R) {
+            case 16: stockham_stage<16, INV>(s, nl, ld, n, Ns, tw); break;
+            case 8: stockham_stage<8, INV>(s, nl, ld, n, Ns, tw); break;
+            case 4: stockham_stage<4, INV>(s, nl, ld, n, Ns, tw); break;
+            case 2: stockham_stage<2, INV>(s, nl, ld, n, Ns, tw); break;
+            case 3: stockham_stage<3, INV>(s, nl, ld, n, Ns, tw); break;
+            case 5: stockham_stage<5, INV>(s, nl, ld, n, Ns, tw); break;
+            case 7: stockham_stage<7, INV>(s, nl, ld, n, Ns, tw); break;
+            default: break;
+        }
+        Ns *= R;
+    }
+}
+
+// rho_w / rho with rho_w = 1 g/cc; 0 where rho <= 0 (declared epilogue).
+__device__ __forceinline__ float density_factor(float rho) { return rho > 0.0f ? 1.0f / rho : 0.0f; }

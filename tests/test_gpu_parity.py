@@ -1,0 +1,251 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerance: the north-star bound for FP32 convolution, max|D - D64| <= 1e-5 *
+max|D64| (``MAX_REL``); the dose-point kernels are FP64 on both sides
+(relative 1e-12).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import conv, dpk
+
+pytestmark = pytest.mark.gpu
+
+MAX_REL = 1e-5
+
+
+def _say(*a):
+    print("[gpu-case]", *a, file=sys.stderr, flush=True)
+
+
+def dev_tensor(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+
+
+def assert_parity(got, ref, tol=MAX_REL, what=""):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, what
+    assert np.isfinite(got).all(), f"{what}: non-finite output"
+    m = conv.error_metrics(got, ref)
+    assert m["max_rel"] <= tol, f"{what}: max_rel={m['max_rel']:.3e} rms_rel={m['rms_rel']:.3e}"
+    return m
+
+
+CONV_CASES = ["ref_small_aniso", "ref_crop", "ref_nonsmooth", "ref_prime", "ref_y90_24", "ref_lu177_even_kernel",
+              "ref_density_scale", "lin_small", "lin_big_kernel", "lin_even_kernel", "lin_density", "ts_trapezoid",
+              "ts_uneven", "ref_single_voxel", "lin_thin"]
+
+
+def run_case(case, algo):
+    from pyvoxeldosimetry_amd import ConvPlan
+    acts = case["acts"]
+    mode = str(case["mode"])
+    plan = ConvPlan(acts.shape[1:], case["kernel"].shape, mode=mode, algo=algo)
+    plan.set_kernel(case["kernel"])
+    dens = dev_tensor(case["density"]) if "density" in case else None
+    scale = float(case["scale"])
+    if "times" in case:
+        out = plan.dose_timeseries([dev_tensor(a) for a in acts], list(case["times"]), density=dens, scale=scale)
+    else:
+        out = plan.dose(dev_tensor(acts[0]), density=dens, scale=scale)
+    res = out.cpu().numpy()
+    plan.close()
+    return res
+
+
+@pytest.mark.parametrize("name", CONV_CASES)
+@pytest.mark.parametrize("algo", ["fft", "direct"])
+def test_golden_conv(golden, name, algo):
+    case = golden[name]
+    if algo == "direct":
+        mode = str(case["mode"])
+        shp = case["acts"].shape[1:]
+        ku = [min(k, n) if mode == "reference" else k for k, n in zip(case["kernel"].shape, shp)]
+        if max(ku) > 16:
+            pytest.skip("kernel too large for the direct path")
+    _say(name, algo)
+    assert_parity(run_case(case, algo), case["expect"], what=f"{name}/{algo}")
+
+
+def test_golden_nan_kernel_propagates(golden):
+    _say("ref_y90_nan")
+    case = golden["ref_y90_nan"]
+    assert np.isnan(case["expect"]).all()
+    for algo in ("fft", "direct"):
+        got = run_case(case, algo)
+        assert np.isnan(got).all(), algo
+
+
+@pytest.mark.parametrize("name", ["dpk_y90_water_fix", "dpk_y90_bone_raw", "dpk_y90_lung_even", "dpk_lu177_water",
+                                  "dpk_lu177_lung", "dpk_lu177_unknown_tissue"])
+def test_golden_dose_point_kernels(golden, name):
+    from pyvoxeldosimetry_amd.dose_kernels import GENERATORS
+    _say(name)
+    c = golden[name]
+    nuc = str(c["nuclide"])
+    gen = GENERATORS[nuc](str(c["tissue"]), fix_centre=bool(c["fix_centre"])) if nuc == "Y90" else \
+        GENERATORS[nuc](str(c["tissue"]))
+    got = gen.generate_kernel(float(c["voxel"]), tuple(int(g) for g in c["grid"]))
+    exp = c["expect"]
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(exp))
+    fin = np.isfinite(exp)
+    np.testing.assert_allclose(got[fin], exp[fin], rtol=1e-12, atol=0)
+
+
+# --- full-size configurations ----------------------------------------------------------
+def test_small_config_64cubed_y90(capsys=None):
+    """64^3 at 4.42 mm, Y-90 11^3 (fixed centre), decay-scaled, reference and linear modes."""
+    from pyvoxeldosimetry_amd import ConvPlan, Y90KernelGenerator
+    from pyvoxeldosimetry_amd.phantoms import small_phantom
+    _say("small config 64^3")
+    p = small_phantom()
+    k = Y90KernelGenerator("water", fix_centre=True).generate_kernel_device(p.voxel_mm, p.kernel_grid)
+    knp = k.cpu().numpy()
+    decay = 2.0 ** (-2.0 / 64.1)
+    a = dev_tensor(p.activity[0])
+    for mode, oracle_fn in (("reference", conv.ref_conv), ("linear", conv.linear_same)):
+        exp = oracle_fn(p.activity[0].astype(np.float64), knp) * decay
+        for algo in ("fft", "direct"):
+            plan = ConvPlan(p.shape, p.kernel_grid, mode=mode, algo=algo)
+            plan.set_kernel(k)
+            got = plan.dose(a, scale=decay).cpu().numpy()
+            m = assert_parity(got, exp, what=f"small/{mode}/{algo}")
+            _say(f"small {mode} {algo} max_rel={m['max_rel']:.2e} rms_rel={m['rms_rel']:.2e}")
+            plan.close()
+
+
+def test_ct_config_256x256x200_lu177_density():
+    """256x256x200 at 2 mm, Lu-177 31^3, density scaling; FP64 oracle of the reference expression."""
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    from pyvoxeldosimetry_amd.phantoms import ct_phantom
+    _say("ct config 256x256x200")
+    p = ct_phantom()
+    k = Lu177KernelGenerator("water").generate_kernel_device(p.voxel_mm, p.kernel_grid)
+    exp = conv.epilogue(conv.ref_conv(p.activity[0].astype(np.float64), k.cpu().numpy()), p.density)
+    plan = ConvPlan(p.shape, p.kernel_grid, mode="reference", algo="fft")
+    plan.set_kernel(k)
+    got = plan.dose(dev_tensor(p.activity[0]), density=dev_tensor(p.density)).cpu().numpy()
+    m = assert_parity(got, exp, what="ct/reference/fft")
+    _say(f"ct reference max_rel={m['max_rel']:.2e} rms_rel={m['rms_rel']:.2e} fft_n={plan.fft_n}")
+
+
+def test_ct_config_linear_mode():
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    from pyvoxeldosimetry_amd.phantoms import ct_phantom
+    _say("ct config linear")
+    p = ct_phantom()
+    k = Lu177KernelGenerator("water").generate_kernel_device(p.voxel_mm, p.kernel_grid)
+    exp = conv.epilogue(conv.linear_same(p.activity[0].astype(np.float64), k.cpu().numpy()), p.density)
+    plan = ConvPlan(p.shape, p.kernel_grid, mode="linear", algo="fft")
+    plan.set_kernel(k)
+    got = plan.dose(dev_tensor(p.activity[0]), density=dev_tensor(p.density)).cpu().numpy()
+    m = assert_parity(got, exp, what="ct/linear/fft")
+    _say(f"ct linear max_rel={m['max_rel']:.2e} fft_n={plan.fft_n}")
+
+
+def test_nonsmooth_reference_fold_against_bruteforce_c():
+    """Prime-sized grid: padded FFT + fold must equal circular brute force (independent C)."""
+    from oracle import brute
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("fold vs C brute force 29x31x37")
+    rng = np.random.default_rng(11)
+    a = rng.random((29, 31, 37)).astype(np.float32)
+    k = rng.random((9, 9, 9))
+    exp = brute.brute_conv3d(a.astype(np.float64), k, "reference")
+    for fft_n in (None, (40, 40, 48)):
+        plan = ConvPlan(a.shape, k.shape, mode="reference", algo="fft", fft_n=fft_n)
+        plan.set_kernel(k)
+        assert_parity(plan.dose(dev_tensor(a)).cpu().numpy(), exp, what=f"fold {fft_n}")
+
+
+# --- properties at full size -------------------------------------------------------------
+def test_delta_returns_kernel_at_full_size():
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    _say("delta 256x256x200")
+    shape = (256, 256, 200)
+    k = Lu177KernelGenerator("water").generate_kernel_device(2.0, (31, 31, 31))
+    plan = ConvPlan(shape, (31, 31, 31), mode="reference", algo="fft")
+    plan.set_kernel(k)
+    a = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    a[100, 17, 190] = 1.0
+    got = plan.dose(a).cpu().numpy()
+    exp = np.roll(conv.pad_or_crop(k.cpu().numpy(), shape), (100, 17, 190), axis=(0, 1, 2))
+    assert_parity(got, exp, what="delta")
+
+
+def test_linearity_at_full_size():
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    _say("linearity 256x256x200")
+    shape = (256, 256, 200)
+    k = Lu177KernelGenerator("water").generate_kernel_device(2.0, (31, 31, 31))
+    plan = ConvPlan(shape, (31, 31, 31), mode="reference", algo="fft")
+    plan.set_kernel(k)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    a = torch.rand(shape, device="cuda", generator=g)
+    b = torch.rand(shape, device="cuda", generator=g)
+    da, db = plan.dose(a), plan.dose(b)
+    dab = plan.dose_weighted([a, b], [2.0, -0.5])
+    ref = 2.0 * da.double() - 0.5 * db.double()
+    err = (dab.double() - ref).abs().max().item() / ref.abs().max().item()
+    assert err < MAX_REL
+    # constant input -> constant * sum(kernel)
+    c = plan.dose(torch.full(shape, 3.0, device="cuda")).double()
+    exp = 3.0 * k.sum().item()
+    assert (c - exp).abs().max().item() / exp < MAX_REL
+
+
+def test_repeat_calls_are_deterministic_and_out_param():
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("determinism")
+    rng = np.random.default_rng(5)
+    a = dev_tensor(rng.random((40, 36, 30)))
+    plan = ConvPlan(a.shape, (7, 7, 7), algo="fft")
+    plan.set_kernel(rng.random((7, 7, 7)))
+    o1 = plan.dose(a)
+    o2 = torch.empty_like(a)
+    plan.dose(a, out=o2)
+    assert torch.equal(o1, o2)
+
+
+def test_weighted_sum_kernel():
+    import torch
+    from pyvoxeldosimetry_amd.core.activity_sampler import weighted_sum_device
+    _say("weighted sum")
+    rng = np.random.default_rng(2)
+    maps = [rng.random((17, 9, 13)).astype(np.float32) for _ in range(5)]
+    w = [0.5, -1.25, 3.0, 0.0, 7.5]
+    got = weighted_sum_device([dev_tensor(m) for m in maps], w).cpu().numpy()
+    exp = sum(wi * m.astype(np.float64) for wi, m in zip(w, maps))
+    np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-5)
+    # unaligned tail path
+    base = torch.from_numpy(np.arange(1001, dtype=np.float32)).cuda()
+    got = weighted_sum_device([base[1:], base[1:]], [1.0, 2.0]).cpu().numpy()
+    np.testing.assert_allclose(got, 3.0 * np.arange(1, 1001), rtol=1e-7)
+
+
+def test_plan_rejects_bad_inputs():
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("plan validation")
+    with pytest.raises(ValueError):
+        ConvPlan((8, 8, 8), (3, 3, 3), fft_n=(9, 8, 8))        # aliases in reference mode
+    with pytest.raises(ValueError):
+        ConvPlan((8, 8, 8), (3, 3, 3), fft_n=(11, 8, 8))       # not 7-smooth
+    with pytest.raises(NotImplementedError):
+        ConvPlan((8, 8, 8), (31, 3, 3), mode="linear", algo="direct")
+    plan = ConvPlan((8, 8, 8), (3, 3, 3))
+    a = torch.zeros((8, 8, 8), device="cuda")
+    with pytest.raises(ValueError, match="no kernel"):
+        plan.dose(a)
+    plan.set_kernel(np.ones((3, 3, 3)))
+    with pytest.raises(ValueError):
+        plan.dose(torch.zeros((8, 8, 9), device="cuda"))
+    with pytest.raises(ValueError):
+        plan.dose(a.double())

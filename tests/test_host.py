@@ -1,0 +1,147 @@
+"""CPU tests of the host logic and the C ABI surface (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "pvd.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pvd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pyvoxeldosimetry_amd import _native
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    missing = [f for f in header_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_binding_types_every_declared_symbol():
+    from pyvoxeldosimetry_amd import _native
+    assert sorted(_native.SIGNATURES) == header_functions()
+
+
+def test_binding_loads_and_reports_version_without_gpu():
+    from pyvoxeldosimetry_amd import _native
+    lib = _native.load()
+    assert lib.pvd_version() >= 100
+
+
+def test_struct_layouts_match_header():
+    from pyvoxeldosimetry_amd import _native as nat
+    assert ctypes.sizeof(nat.PlanParams) == 4 * 12
+    assert ctypes.sizeof(nat.DpkTerm) == 24
+    assert ctypes.sizeof(nat.PlanInfo) == 8 * 4 + 8
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from pyvoxeldosimetry_amd import ConvPlan
+    with pytest.raises(RuntimeError, match="GPU"):
+        ConvPlan((8, 8, 8), (3, 3, 3))
+
+
+def test_missing_library_raises(tmp_path):
+    from pyvoxeldosimetry_amd import _native
+    with pytest.raises(_native.NativeLibraryMissing):
+        _native.load(str(tmp_path / "nope.so"))
+
+
+# --- reference error behaviour (raised before any device work) -------------------------
+def test_non_string_radionuclide_is_type_error():
+    from pyvoxeldosimetry_amd import KernelConvolutionCalculator
+    with pytest.raises(TypeError, match="Radionuclide must be a string"):
+        KernelConvolutionCalculator(90, "water")
+
+
+def test_unknown_nuclide_is_value_error():
+    from pyvoxeldosimetry_amd import KernelFactory
+    with pytest.raises(ValueError, match="Unsupported nuclide: F18"):
+        KernelFactory().get_kernel("F18", "water")
+
+
+def test_dose_calculator_rejects_unknown_method():
+    from pyvoxeldosimetry_amd import DoseCalculator
+    with pytest.raises(ValueError, match="Unsupported calculation method"):
+        DoseCalculator("Y90", method="magic")
+
+
+def test_activity_sampler_checks():
+    from pyvoxeldosimetry_amd import ActivitySampler
+    with pytest.raises(ValueError, match="Half-life must be positive"):
+        ActivitySampler(0.0)
+    with pytest.raises(ValueError, match="Invalid time units"):
+        ActivitySampler(1.0, units="days")
+    s = ActivitySampler(64.1)
+    np.testing.assert_allclose(s.sample_timepoints(0, 10, 5), np.linspace(0, 10, 5))
+
+
+def test_trapezoid_weights_match_numpy():
+    from pyvoxeldosimetry_amd.core.activity_sampler import trapezoid_weights
+    t = [1.0, 4.0, 24.0, 48.0, 96.0, 168.0]
+    x = np.random.default_rng(0).random(len(t))
+    assert np.dot(trapezoid_weights(t), x) == pytest.approx(np.trapezoid(x, t), rel=1e-14)
+    assert trapezoid_weights([5.0]) == [0.0]
+
+
+# --- workloads ---------------------------------------------------------------------------
+def test_spherical_activity_matches_loop_definition():
+    from pyvoxeldosimetry_amd.phantoms import create_spherical_activity
+    a = create_spherical_activity((12, 11, 10), (5, 6, 4), 3, 7.0)
+    exp = np.zeros((12, 11, 10))
+    for x in range(12):
+        for y in range(11):
+            for z in range(10):
+                if (x - 5) ** 2 + (y - 6) ** 2 + (z - 4) ** 2 <= 9:
+                    exp[x, y, z] = 7.0
+    np.testing.assert_array_equal(a, exp)
+
+
+def test_small_phantom_is_seeded_and_sized():
+    from pyvoxeldosimetry_amd.phantoms import bq_per_voxel, small_phantom
+    p1, p2 = small_phantom(1), small_phantom(1)
+    np.testing.assert_array_equal(p1.activity[0], p2.activity[0])
+    assert p1.shape == (64, 64, 64)
+    assert bq_per_voxel(1.0, 4.42) == pytest.approx(86350.888, rel=1e-6)
+    inside = p1.activity[0][p1.activity[0] > 0]
+    assert abs(inside.mean() / bq_per_voxel(1.0, 4.42) - 1) < 0.01
+    assert 0.03 < inside.std() / inside.mean() < 0.07      # 5 % Poisson noise
+
+
+def test_batch_specs_ranges():
+    from pyvoxeldosimetry_amd.phantoms import batch_specs
+    specs = batch_specs()
+    assert len(specs) == 48
+    for s in specs:
+        assert 96 <= s.shape[0] <= 384 and 96 <= s.shape[1] <= 384 and 96 <= s.shape[2] <= 320
+        assert 20 <= s.t_eff_h <= 150
+        assert (s.nuclide, s.kernel_grid) in (("Y90", (11, 11, 11)), ("Lu177", (31, 31, 31)))
+
+
+def test_ct_phantom_density_classes():
+    from pyvoxeldosimetry_amd.phantoms import ct_phantom
+    p = ct_phantom()
+    vals = set(np.unique(p.density).astype(np.float64).round(3).tolist())
+    assert vals == {0.3, 1.0, 1.85}
+    assert p.activity[0].max() > 10 * np.median(p.activity[0][p.activity[0] > 0])
+
+
+def test_autotune_candidate_lengths_do_not_alias():
+    from pyvoxeldosimetry_amd.autotune import candidate_lengths, _smooth
+    for n, k, mode in [(97, 11, "reference"), (256, 31, "reference"), (200, 31, "linear"), (5, 64, "reference")]:
+        for p in candidate_lengths(n, k, mode):
+            assert _smooth(p)
+            if mode == "reference":
+                assert p == n or p >= n + min(k, n) - 1
+            else:
+                assert p >= n + k // 2

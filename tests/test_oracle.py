@@ -1,0 +1,207 @@
+"""Pins the FP64 oracle (CPU only) -- brute force, closed forms, library routines."""
+import numpy as np
+import pytest
+import scipy.signal
+
+from oracle import brute, conv, dpk
+from tests.conftest import load_golden
+
+RNG = np.random.default_rng(7)
+
+SHAPES = [((8, 9, 10), (3, 5, 4)), ((6, 7, 5), (9, 7, 5)), ((12, 10, 7), (5, 5, 5)),
+          ((5, 4, 3), (5, 5, 5)), ((1, 6, 8), (3, 3, 3)), ((7, 7, 7), (1, 1, 1))]
+
+
+@pytest.mark.parametrize("shape,ks", SHAPES)
+def test_ref_conv_matches_bruteforce_python_and_c(shape, ks):
+    a = RNG.random(shape)
+    k = RNG.random(ks)
+    r = conv.ref_conv(a, k)
+    np.testing.assert_allclose(r, conv.brute_circular(a, k), rtol=0, atol=1e-12 * np.abs(r).max())
+    np.testing.assert_allclose(r, brute.brute_conv3d(a, k, "reference"), rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("shape,ks", SHAPES)
+def test_linear_same_matches_bruteforce_python_and_c(shape, ks):
+    a = RNG.random(shape)
+    k = RNG.random(ks)
+    r = conv.linear_same(a, k)
+    np.testing.assert_allclose(r, conv.brute_linear_same(a, k), rtol=0, atol=1e-12 * np.abs(r).max())
+    np.testing.assert_allclose(r, brute.brute_conv3d(a, k, "linear"), rtol=0, atol=1e-12 * np.abs(r).max())
+
+
+def test_linear_same_matches_scipy_for_odd_kernels():
+    a = RNG.random((11, 9, 13))
+    k = RNG.random((5, 3, 7))
+    np.testing.assert_allclose(conv.linear_same(a, k),
+                               scipy.signal.convolve(a, k, mode="same", method="direct"), atol=1e-12)
+
+
+def test_linear_same_even_kernel_is_full_conv_sliced_at_k_half():
+    a = RNG.random((9, 8, 7))
+    k = RNG.random((4, 2, 6))
+    full = scipy.signal.convolve(a, k, mode="full", method="direct")
+    c = [s // 2 for s in k.shape]
+    exp = full[c[0]:c[0] + 9, c[1]:c[1] + 8, c[2]:c[2] + 7]
+    np.testing.assert_allclose(conv.linear_same(a, k), exp, atol=1e-12)
+
+
+def test_delta_returns_rolled_padded_kernel():
+    n = (10, 12, 9)
+    k = RNG.random((5, 5, 5))
+    a = np.zeros(n)
+    j = (3, 7, 8)
+    a[j] = 1.0
+    exp = np.roll(conv.pad_or_crop(k, n), j, axis=(0, 1, 2))
+    np.testing.assert_allclose(conv.ref_conv(a, k), exp, atol=1e-13)
+
+
+def test_delta_with_cropped_kernel():
+    n = (4, 6, 3)
+    k = RNG.random((7, 5, 5))
+    a = np.zeros(n)
+    a[0, 0, 0] = 1.0
+    exp = np.zeros(n)
+    exp[:, :5, :] = k[:4, :5, :3]          # cropped to [0:N] on x and z, zero-padded on y
+    np.testing.assert_allclose(conv.ref_conv(a, k), exp, atol=1e-13)
+
+
+def test_constant_input_gives_constant_times_kernel_sum():
+    n = (9, 8, 10)
+    k = RNG.random((5, 9, 3))
+    a = np.full(n, 2.5)
+    exp = 2.5 * conv.pad_or_crop(k, n).sum()
+    np.testing.assert_allclose(conv.ref_conv(a, k), exp, rtol=1e-13)
+
+
+def test_reference_is_centred_circular_rolled_by_half_kernel():
+    a = RNG.random((12, 11, 10))
+    k = RNG.random((5, 7, 3))
+    rolled = np.roll(conv.circular_same(a, k), [s // 2 for s in k.shape], axis=(0, 1, 2))
+    np.testing.assert_allclose(conv.ref_conv(a, k), rolled, atol=1e-12)
+
+
+def test_sphere_dose_peak_lands_at_centre_plus_half_kernel():
+    # centred sphere + symmetric kernel: reference-mode peak moves by K//2 on each axis
+    from pyvoxeldosimetry_amd.phantoms import create_spherical_activity
+    a = create_spherical_activity((64, 64, 64), (32, 32, 32), 10, 1.0)
+    k = dpk.y90_kernel(4.42, (11, 11, 11), fix_centre=True)
+    d = conv.ref_conv(a, k)
+    # the flat-topped dose is symmetric about 37 on every axis (not 32)
+    for ax in range(3):
+        idx = (2 * 37 - np.arange(64)) % 64          # reflection i -> 74 - i (mod 64)
+        np.testing.assert_allclose(d, np.take(d, idx, axis=ax), atol=1e-10 * d.max())
+    assert d[37, 37, 37] == pytest.approx(d.max(), rel=1e-12)
+
+
+def test_linearity_and_shift_equivariance():
+    a, b = RNG.random((8, 8, 8)), RNG.random((8, 8, 8))
+    k = RNG.random((3, 3, 3))
+    np.testing.assert_allclose(conv.ref_conv(2 * a - 3 * b, k), 2 * conv.ref_conv(a, k) - 3 * conv.ref_conv(b, k),
+                               atol=1e-12)
+    np.testing.assert_allclose(conv.ref_conv(np.roll(a, (1, 2, 3), (0, 1, 2)), k),
+                               np.roll(conv.ref_conv(a, k), (1, 2, 3), (0, 1, 2)), atol=1e-12)
+
+
+def test_trapezoid_matches_numpy_trapezoid():
+    t = [0.0, 2.0, 5.0, 11.0, 24.0]
+    maps = [RNG.random((4, 5, 6)) for _ in t]
+    exp = np.trapezoid(np.stack(maps), x=t, axis=0)
+    np.testing.assert_allclose(conv.activity_trapezoid(maps, t), exp, rtol=1e-13)
+    w = conv.trapezoid_weights_seconds(t)
+    np.testing.assert_allclose(sum(wi * m for wi, m in zip(w, maps)), exp * 3600.0, rtol=1e-13)
+
+
+def test_absorbed_dose_equals_conv_of_trapezoid_activity():
+    t = [0.0, 24.0, 48.0, 96.0]
+    maps = [RNG.random((6, 6, 6)) for _ in t]
+    k = RNG.random((3, 3, 3))
+    d1 = conv.ref_absorbed_dose(maps, t, k)
+    d2 = conv.ref_conv(np.trapezoid(np.stack(maps), x=t, axis=0) * 3600.0, k)
+    np.testing.assert_allclose(d1, d2, rtol=1e-12)
+
+
+def test_absorbed_dose_linear_activity_is_exact_integral():
+    # A(t) = a + b t: the trapezoid is exact
+    a, b = RNG.random((5, 5, 5)), RNG.random((5, 5, 5))
+    k = RNG.random((3, 3, 3))
+    t = [0.0, 3.0, 10.0]
+    maps = [a + b * ti for ti in t]
+    integral = (a * 10.0 + b * 50.0) * 3600.0
+    np.testing.assert_allclose(conv.ref_absorbed_dose(maps, t, k), conv.ref_conv(integral, k), rtol=1e-12)
+
+
+def test_density_epilogue_constant_maps():
+    a = RNG.random((6, 7, 8))
+    k = RNG.random((3, 3, 3))
+    d = conv.ref_conv(a, k)
+    np.testing.assert_allclose(conv.epilogue(d, np.ones(a.shape)), d)
+    np.testing.assert_allclose(conv.epilogue(d, np.full(a.shape, 2.0), 3.0), d * 1.5)
+    z = np.ones(a.shape)
+    z[1, 2, 3] = 0.0
+    assert conv.epilogue(d, z)[1, 2, 3] == 0.0
+
+
+def test_error_metrics():
+    r = np.array([0.0, 2.0, -4.0])
+    m = conv.error_metrics(r + np.array([0.0, 0.4, 0.0]), r)
+    assert m["max_rel"] == pytest.approx(0.1)
+    assert m["rms_rel"] == pytest.approx(np.sqrt(0.16 / 3) / 4)
+
+
+# --- dose-point kernels ------------------------------------------------------------------
+def test_y90_raw_kernel_has_single_nan_at_grid_centre():
+    k = dpk.y90_kernel(1.0, (64, 64, 64))
+    assert np.isnan(k).sum() == 1 and np.isnan(k[32, 32, 32])
+    # shipped cached_kernels/Y90_water_xy.png: 64x64 slice, colour bar max ~0.90, blank centre
+    assert abs(np.nanmax(k) - 0.90) < 0.005
+
+
+def test_y90_fixed_centre_value_is_rho_spr_factor():
+    for tissue, exp in [("water", 1.0), ("bone", 1.85 * 1.15 * 1.15), ("lung", 0.26 * 1.04 * 1.04)]:
+        k = dpk.y90_kernel(2.0, (5, 5, 5), tissue, fix_centre=True)
+        assert k[2, 2, 2] == pytest.approx(exp, rel=1e-14)
+
+
+def test_y90_far_field_is_bremsstrahlung_only():
+    v = 5.0
+    k = dpk.y90_kernel(v, (21, 21, 21), fix_centre=True)
+    # beyond R = 11*2.28^1.5 = 37.87 mm only the photon term remains
+    i, j, l = 10 + 8, 10 + 3, 10
+    r = v * np.sqrt(8 ** 2 + 3 ** 2)
+    assert r > 11 * 2.28 ** 1.5
+    assert k[i, j, l] == pytest.approx(dpk.photon_point_source(r, 0.015, 0.096), rel=1e-13)
+
+
+def test_lu177_centre_is_sum_of_abundances_and_tail_is_gamma():
+    k = dpk.lu177_kernel(2.0, (31, 31, 31))
+    assert k[15, 15, 15] == pytest.approx(sum(dpk.LU177_BETA_ABUND), rel=1e-14)
+    r = 2.0 * np.sqrt(4 ** 2 + 1 + 0)   # beyond every beta range (max 1.76 mm)
+    exp = sum(dpk.photon_point_source(r, inten, 0.096 * (0.2 / e) ** 3.2) for e, inten in dpk.LU177_GAMMAS)
+    assert k[19, 16, 15] == pytest.approx(exp, rel=1e-13)
+
+
+def test_kernels_are_isotropic_on_odd_grids():
+    for k in (dpk.lu177_kernel(1.3, (9, 9, 9), "bone"), dpk.y90_kernel(3.0, (7, 7, 7), "lung", True)):
+        np.testing.assert_allclose(k, k[::-1, :, :], rtol=1e-14)
+        np.testing.assert_allclose(k, np.transpose(k, (2, 0, 1)), rtol=1e-14)
+
+
+def test_even_grid_centre_is_floor_half():
+    k = dpk.lu177_kernel(1.0, (6, 8, 4))
+    assert np.unravel_index(np.argmax(k), k.shape) == (3, 4, 2)
+
+
+# --- fixtures stay in sync with the oracle -----------------------------------------------
+def test_golden_fixtures_reproduce():
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "make_golden", os.path.join(os.path.dirname(__file__), "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    fresh = mg.build_cases()
+    stored = load_golden()
+    assert set(fresh) == set(stored)
+    for name, case in fresh.items():
+        np.testing.assert_array_equal(np.asarray(case["expect"]), stored[name]["expect"], err_msg=name)
