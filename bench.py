@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 error check")
     ap.add_argument("--profile-reps", type=int, default=50)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 10 device copies of 256 MiB (reference for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -93,6 +95,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "copies": 10}))
+        return
 
     from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
     from pyvoxeldosimetry_amd.phantoms import ct_phantom

@@ -57,7 +57,8 @@ def pad_or_crop(kernel: np.ndarray, shape) -> np.ndarray:
 def ref_conv(activity: np.ndarray, kernel: np.ndarray) -> np.ndarray:
     """Verbatim restatement of ``kernel_convolution.py:71-74`` (complex128)."""
     a = np.asarray(activity)
-    return np.fft.ifftn(np.fft.fftn(a) * np.fft.fftn(kernel, a.shape)).real
+    # axes spelled out only to silence NumPy 2's deprecation; identical for 3-D input
+    return np.fft.ifftn(np.fft.fftn(a) * np.fft.fftn(kernel, a.shape, axes=(0, 1, 2))).real
 
 
 def brute_circular(activity: np.ndarray, kernel: np.ndarray) -> np.ndarray:
@@ -125,8 +126,8 @@ def linear_same(activity: np.ndarray, kernel: np.ndarray) -> np.ndarray:
     a = np.asarray(activity, dtype=np.float64)
     k = np.asarray(kernel, dtype=np.float64)
     full = [a.shape[i] + k.shape[i] - 1 for i in range(3)]
-    spec = np.fft.rfftn(a, full) * np.fft.rfftn(k, full)
-    lin = np.fft.irfftn(spec, full)
+    spec = np.fft.rfftn(a, full, axes=(0, 1, 2)) * np.fft.rfftn(k, full, axes=(0, 1, 2))
+    lin = np.fft.irfftn(spec, full, axes=(0, 1, 2))
     c = [s // 2 for s in k.shape]
     return lin[c[0]:c[0] + a.shape[0], c[1]:c[1] + a.shape[1], c[2]:c[2] + a.shape[2]]
 

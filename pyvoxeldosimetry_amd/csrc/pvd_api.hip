@@ -25,7 +25,7 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes);
 hipError_t pvd_launch_dpk(double v, const int g[3], const pvd_dpk_term *terms, int nterms, double factor,
                           double *out, hipStream_t st);
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
-                               const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st);
+                               int Hp, const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st);
 size_t pvd_direct_lds_bytes(const int k[3]);
 hipError_t pvd_direct_set_lds_limit(size_t bytes);
 hipError_t pvd_launch_wsum(const float *const *maps, const float *w, int T, size_t n, float *out, hipStream_t st);
@@ -212,12 +212,14 @@ int axis_setup(pvd_plan *p) {
     return PVD_OK;
 }
 
-// lines per tile and block size so that lines*n <= 16*block and LDS fits
+// power-of-two lines per tile and a block size so that lines*n <= 16*block and
+// the tile plus twiddles fit the LDS budget
 void pick_tile(int n, int target, int extra_ld, int *lines, int *block, size_t *lds) {
     const size_t cap_elems = kFftLdsCap / sizeof(float2);
-    int l = target;
+    int l = 1;
+    while (l * 2 <= target) l *= 2;
     const int twn = round_up(n, 8);
-    while (l > 1 && ((size_t)twn + (size_t)n * (l + extra_ld) > cap_elems || l * n > 16 * 1024)) --l;
+    while (l > 1 && ((size_t)twn + (size_t)n * (l + extra_ld) > cap_elems || l * n > 16 * 1024)) l /= 2;
     *lines = l;
     int b = round_up((l * n + 15) / 16, 64);
     b = std::max(64, std::min(1024, b));
@@ -225,25 +227,31 @@ void pick_tile(int n, int target, int extra_ld, int *lines, int *block, size_t *
     *lds = sizeof(float2) * ((size_t)twn + (size_t)n * (l + extra_ld));
 }
 
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << (r + 1)) <= v) ++r;
+    return r;
+}
+
 int fft_setup(pvd_plan *p) {
     pvd_vol_d &v = p->vol;
     const int Nx = p->N[0], Ny = p->N[1];
     const int Px = p->P[0], Py = p->P[1], Pz = p->P[2];
     v.H = Pz / 2 + 1;
+    v.Hp = round_up(v.H, 16);
     int lz, bz; size_t ldsz;
     pick_tile(Pz, 16, 1, &lz, &bz, &ldsz);
     int tky, by; size_t ldsy;
-    pick_tile(Py, std::min(16, v.H), 0, &tky, &by, &ldsy);
+    pick_tile(Py, 16, 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
     pick_tile(Px, 16, 0, &tkx, &bx, &ldsx);
-    v.lz = lz;
-    v.tky = tky;
-    v.tkx = tkx;
+    v.llz = ilog2(lz);
+    v.ltky = ilog2(tky);
+    v.ltkx = ilog2(tkx);
     const long long nlines = (long long)Nx * Ny;
     const long long tz = (nlines + 2 * lz - 1) / (2 * lz);
     const long long ty = (long long)Nx * ((v.H + tky - 1) / tky);
-    const long long plane = (long long)Py * v.H;
-    const long long tx = (plane + tkx - 1) / tkx;
+    const long long tx = (long long)Py * ((v.H + tkx - 1) / tkx);
     if (tz > 0x7fffffffLL || ty > 0x7fffffffLL || tx > 0x7fffffffLL) return fail(PVD_ENOTSUP, "grid too large");
     p->cfg[PVD_PASS_FZ] = {(int)tz, bz, ldsz};
     p->cfg[PVD_PASS_IZ] = {(int)tz, bz, ldsz};
@@ -260,8 +268,8 @@ int fft_setup(pvd_plan *p) {
         if (rc) return rc;
         v.ax[a].tw = tw;
     }
-    const size_t work_elems = (size_t)Nx * Py * v.H;
-    const size_t spec_elems = (size_t)Px * Py * v.H;
+    const size_t work_elems = (size_t)Nx * Py * v.Hp;
+    const size_t spec_elems = (size_t)Px * Py * v.Hp;
     PVD_HIP(hipMalloc(&p->work, sizeof(float2) * work_elems), "allocate FFT workspace");
     PVD_HIP(hipMalloc(&p->spec, sizeof(float2) * spec_elems), "allocate kernel spectrum");
     p->bytes = sizeof(float2) * (work_elems + spec_elems);
@@ -269,6 +277,8 @@ int fft_setup(pvd_plan *p) {
     v.spec = p->spec;
     return PVD_OK;
 }
+
+bool aligned16(const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; }
 
 double fft_cost_model(const pvd_plan *p) {
     double vol = (double)p->N[0] * p->N[1] * p->N[2];
@@ -325,6 +335,10 @@ int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const
     v.dens = dens;
     v.out = out;
     v.scale = (float)scale;
+    // float4 access when every tile's first voxel is 16-byte aligned
+    bool vec = ((2LL << v.llz) * p->N[2]) % 4 == 0 && aligned16(out) && aligned16(dens);
+    for (int t = 0; t < T; ++t) vec = vec && aligned16(acts[t]);
+    v.vec_io = vec ? 1 : 0;
     return launch_fft(p, v, st);
 }
 
@@ -454,7 +468,7 @@ int pvd_plan_set_kernel(pvd_plan *p, const double *kernel, void *stream) {
     PVD_HIP(hipMalloc(&T1, sizeof(double2) * n1), "allocate spectrum scratch");
     hipError_t e = hipMalloc(&T2, sizeof(double2) * n2);
     if (e != hipSuccess) { (void)hipFree(T1); return hip_fail(e, "allocate spectrum scratch"); }
-    e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, kernel, T1, T2, p->spec, st);
+    e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, p->vol.Hp, kernel, T1, T2, p->spec, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(T1);
     (void)hipFree(T2);

@@ -1,18 +1,22 @@
 // FFT dose-convolution passes (gfx950).
 //
 // The dose is computed as five descriptor-driven passes over an in-place
-// complex workspace work[Nx][Py][H] (H = Pz/2+1):
+// complex workspace work[Nx][Py][Hp] (H = Pz/2+1 used columns, rows padded to
+// Hp = 16*ceil(H/16) complex = whole 128-byte lines):
 //   FZ  z: weighted sum of the T activity maps, zero-pad, two-for-one real FFT
 //   FY  y: forward FFT of every (x, kz) column
 //   X   x: forward FFT, times the kernel spectrum, inverse FFT, fold/roll map
 //   IY  y: inverse FFT, fold/roll map
 //   IZ  z: two-for-one inverse real FFT, fold/roll map, scale * rho_w/rho, store
-// Every pass stages its lines in LDS (k-major: element (line l, index k) at
-// s[k*ld + l]) so global loads and stores are coalesced along the contiguous
-// axis of the workspace and the Stockham stages run out of LDS.
+// Every pass stages its lines in LDS (element (line l, index k) at s[k*ld + l])
+// so global accesses are coalesced along the contiguous axis; all global loads
+// of a tile are issued before any is consumed (unrolled, predicated) so each
+// thread keeps many requests in flight.
 #include "pvd_internal.h"
 
 namespace {
+
+constexpr int kBatch = 8;   // global loads issued per thread before the first is consumed
 
 __device__ __forceinline__ int wrap_add(int a, int b, int n) {  // (a + b) mod n, a,b in [0, n)
     int r = a + b;
@@ -23,34 +27,133 @@ __device__ __forceinline__ void load_twiddles(float2 *tw, const pvd_axis_d &ax) 
     for (int i = threadIdx.x; i < ax.n; i += blockDim.x) tw[i] = ax.tw[i];
 }
 
+// Load rows [0, rows_in) x columns [0, nc) of a strided tile into LDS
+// s[k*nl + l] (zero elsewhere up to n rows); row k starts at g + k*pitch.
+__device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const float2 *__restrict__ g, long long pitch,
+                                              int n, int rows_in, int lnl, int nc) {
+    const int total = n << lnl;
+    const int mask = (1 << lnl) - 1;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+        float2 v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int k = e >> lnl, l = e & mask;
+            v[u] = (e < total && k < rows_in && l < nc) ? g[k * pitch + l] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < total) s[e] = v[u];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
-// holds 2*lz consecutive real lines paired into lz complex lines.
+// holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1).
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &az = v.ax[2];
-    const int Nz = az.N, Pz = az.n, H = v.H, LZ = v.lz, ld = LZ + 1;
+    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
     const int Ny = v.ax[1].N, Py = v.ax[1].n;
     const long long nlines = (long long)v.ax[0].N * Ny;
-    const long long L0 = (long long)tile * 2 * LZ;
-    const int nreal = (int)min((long long)(2 * LZ), nlines - L0);
+    const long long L0 = (long long)tile * 2 * nl;
+    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     load_twiddles(tw, az);
-    for (int e = threadIdx.x; e < Pz * ld; e += blockDim.x) s[e] = make_float2(0.f, 0.f);
-    __syncthreads();
+    // zero the padding: rows z >= Nz of every line, and the unused line pairs
+    for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
+    const int npair = (nreal + 1) >> 1;
+    if (npair < nl || (nreal & 1)) {
+        for (int e = threadIdx.x; e < Nz * ld; e += blockDim.x) {
+            const int q = e % ld;
+            if (q >= nreal / 2) s[e] = make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+    }
+    // weighted sum of the T maps over one contiguous chunk of nreal*Nz floats
     const long long base = L0 * Nz;
     const int total = nreal * Nz;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int ln = e / Nz;
-        const int z = e - ln * Nz;
-        float a = v.w[0] * v.act[0][base + e];
-        for (int t = 1; t < v.T; ++t) a = fmaf(v.w[t], v.act[t][base + e], a);
-        float *p = reinterpret_cast<float *>(&s[z * ld + (ln >> 1)]);
-        p[ln & 1] = a;
+    const float inv_nz = 1.0f / (float)Nz;
+    if (v.vec_io) {
+        const int n4 = total >> 2;      // base is a multiple of 4 floats (2*nl*Nz*tile)
+        for (int i0 = threadIdx.x; i0 < n4; i0 += 8 * blockDim.x) {
+            float4 a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * blockDim.x;
+                a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (i < n4) {
+                    const float4 x = reinterpret_cast<const float4 *>(v.act[0] + base)[i];
+                    a[u] = make_float4(v.w[0] * x.x, v.w[0] * x.y, v.w[0] * x.z, v.w[0] * x.w);
+                }
+            }
+            for (int t = 1; t < v.T; ++t) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = i0 + u * blockDim.x;
+                    if (i < n4) {
+                        const float4 x = reinterpret_cast<const float4 *>(v.act[t] + base)[i];
+                        a[u].x = fmaf(v.w[t], x.x, a[u].x);
+                        a[u].y = fmaf(v.w[t], x.y, a[u].y);
+                        a[u].z = fmaf(v.w[t], x.z, a[u].z);
+                        a[u].w = fmaf(v.w[t], x.w, a[u].w);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * blockDim.x;
+                if (i < n4) {
+                    const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+                    int e = 4 * i;
+                    int ln = fast_div(e, Nz, inv_nz);
+                    int z = e - ln * Nz;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        reinterpret_cast<float *>(&s[z * ld + (ln >> 1)])[ln & 1] = av[c];
+                        if (++z == Nz) { z = 0; ++ln; }
+                    }
+                }
+            }
+        }
+        for (int e = 4 * n4 + threadIdx.x; e < total; e += blockDim.x) {   // tail
+            float a = v.w[0] * v.act[0][base + e];
+            for (int t = 1; t < v.T; ++t) a = fmaf(v.w[t], v.act[t][base + e], a);
+            const int ln = fast_div(e, Nz, inv_nz);
+            reinterpret_cast<float *>(&s[(e - ln * Nz) * ld + (ln >> 1)])[ln & 1] = a;
+        }
+    } else {
+        for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
+            float a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * blockDim.x;
+                a[u] = (e < total) ? v.w[0] * v.act[0][base + e] : 0.f;
+            }
+            for (int t = 1; t < v.T; ++t) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = e0 + u * blockDim.x;
+                    if (e < total) a[u] = fmaf(v.w[t], v.act[t][base + e], a[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < total) {
+                    const int ln = fast_div(e, Nz, inv_nz);
+                    reinterpret_cast<float *>(&s[(e - ln * Nz) * ld + (ln >> 1)])[ln & 1] = a[u];
+                }
+            }
+        }
     }
     __syncthreads();
-    fft_lines<false>(s, LZ, ld, az, tw);
+    fft_lines<false>(s, lnl, ld, az, tw);
     // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i)
+    const float inv_h = 1.0f / (float)H;
     for (int e = threadIdx.x; e < nreal * H; e += blockDim.x) {
-        const int ln = e / H;
+        const int ln = fast_div(e, H, inv_h);
         const int k = e - ln * H;
         const int q = ln >> 1;
         const float2 zk = s[k * ld + q];
@@ -59,135 +162,197 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                                   : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         const long long L = L0 + ln;
         const long long x = L / Ny, y = L - x * Ny;
-        v.work[(x * Py + y) * H + k] = r;
+        v.work[(x * Py + y) * Hp + k] = r;
     }
 }
 
 // ---------------------------------------------------------------------------------
-// FY / IY: tile = (x, block of tky consecutive kz columns); lines run along y.
+// FY / IY: tile = (x, block of 2^ltky consecutive kz columns); lines run along y.
 template <bool INV>
 __device__ void pass_y(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &ay = v.ax[1];
-    const int Ny = ay.N, Py = ay.n, H = v.H, TK = v.tky;
-    const int ntc = (H + TK - 1) / TK;
+    const int Ny = ay.N, Py = ay.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltky, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
     const int x = tile / ntc;
-    const int c0 = (tile - x * ntc) * TK;
+    const int c0 = (tile - x * ntc) << lnl;
     const int nc = min(TK, H - c0);
-    float2 *g = v.work + ((long long)x * Py) * H + c0;
+    float2 *g = v.work + ((long long)x * Py) * Hp + c0;
     load_twiddles(tw, ay);
-    const int rows_in = INV ? Py : Ny;
-    for (int e = threadIdx.x; e < Py * TK; e += blockDim.x) {
-        const int k = e / TK;
-        const int l = e - k * TK;
-        s[e] = (k < rows_in && l < nc) ? g[(long long)k * H + l] : make_float2(0.f, 0.f);
-    }
+    load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
     __syncthreads();
-    fft_lines<INV>(s, TK, TK, ay, tw);
+    fft_lines<INV>(s, lnl, TK, ay, tw);
     if (!INV) {
-        for (int e = threadIdx.x; e < Py * TK; e += blockDim.x) {
-            const int k = e / TK;
-            const int l = e - k * TK;
-            if (l < nc) g[(long long)k * H + l] = s[e];
+        for (int e = threadIdx.x; e < (Py << lnl); e += blockDim.x) {
+            const int k = e >> lnl, l = e & (TK - 1);
+            if (l < nc) g[(long long)k * Hp + l] = s[e];
         }
     } else {
-        for (int e = threadIdx.x; e < Ny * TK; e += blockDim.x) {
-            const int o = e / TK;
-            const int l = e - o * TK;
+        for (int e = threadIdx.x; e < (Ny << lnl); e += blockDim.x) {
+            const int o = e >> lnl, l = e & (TK - 1);
             if (l < nc) {
-                float2 val = s[wrap_add(o, ay.off, Py) * TK + l];
-                if (o < ay.foldn) val = cadd(val, s[((o + Ny + ay.off) % Py) * TK + l]);
-                g[(long long)o * H + l] = val;
+                float2 val = s[(wrap_add(o, ay.off, Py) << lnl) + l];
+                if (o < ay.foldn) val = cadd(val, s[(((o + Ny + ay.off) % Py) << lnl) + l]);
+                g[(long long)o * Hp + l] = val;
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// X: tile = block of tkx consecutive columns of the flattened (y, kz) plane;
+// X: tile = (y, block of 2^ltkx consecutive kz columns) of the (y, kz) plane;
 // forward FFT along x, times the kernel spectrum, inverse FFT, fold/roll map.
 __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &ax = v.ax[0];
-    const int Nx = ax.N, Px = ax.n, TK = v.tkx;
-    const long long plane = (long long)v.ax[1].n * v.H;
-    const long long c0 = (long long)tile * TK;
-    const int nc = (int)min((long long)TK, plane - c0);
-    float2 *g = v.work + c0;
-    const float2 *sp = v.spec + c0;
+    const int Nx = ax.N, Px = ax.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltkx, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
+    const int y = tile / ntc;
+    const int c0 = (tile - y * ntc) << lnl;
+    const int nc = min(TK, H - c0);
+    const long long plane = (long long)v.ax[1].n * Hp;
+    const long long off0 = (long long)y * Hp + c0;
+    float2 *g = v.work + off0;
+    const float2 *sp = v.spec + off0;
     load_twiddles(tw, ax);
-    for (int e = threadIdx.x; e < Px * TK; e += blockDim.x) {
-        const int k = e / TK;
-        const int l = e - k * TK;
-        s[e] = (k < Nx && l < nc) ? g[k * plane + l] : make_float2(0.f, 0.f);
+    load_col_tile(s, g, plane, Px, Nx, lnl, nc);
+    __syncthreads();
+    fft_lines<false>(s, lnl, TK, ax, tw);
+    const int total = Px << lnl;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+        float2 sv[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * blockDim.x;
+            const int k = e >> lnl, l = e & (TK - 1);
+            sv[u] = (e < total && l < nc) ? sp[k * plane + l] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < total) s[e] = cmul(s[e], sv[u]);
+        }
     }
     __syncthreads();
-    fft_lines<false>(s, TK, TK, ax, tw);
-    for (int e = threadIdx.x; e < Px * TK; e += blockDim.x) {
-        const int k = e / TK;
-        const int l = e - k * TK;
-        if (l < nc) s[e] = cmul(s[e], sp[k * plane + l]);
-    }
-    __syncthreads();
-    fft_lines<true>(s, TK, TK, ax, tw);
-    for (int e = threadIdx.x; e < Nx * TK; e += blockDim.x) {
-        const int o = e / TK;
-        const int l = e - o * TK;
+    fft_lines<true>(s, lnl, TK, ax, tw);
+    for (int e = threadIdx.x; e < (Nx << lnl); e += blockDim.x) {
+        const int o = e >> lnl, l = e & (TK - 1);
         if (l < nc) {
-            float2 val = s[wrap_add(o, ax.off, Px) * TK + l];
-            if (o < ax.foldn) val = cadd(val, s[((o + Nx + ax.off) % Px) * TK + l]);
+            float2 val = s[(wrap_add(o, ax.off, Px) << lnl) + l];
+            if (o < ax.foldn) val = cadd(val, s[(((o + Nx + ax.off) % Px) << lnl) + l]);
             g[o * plane + l] = val;
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// IZ: tile = 2*lz consecutive output lines (x, y); inverse two-for-one real FFT.
+// IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &az = v.ax[2];
-    const int Nz = az.N, Pz = az.n, H = v.H, LZ = v.lz, ld = LZ + 1;
+    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
     const int Ny = v.ax[1].N, Py = v.ax[1].n;
     const long long nlines = (long long)v.ax[0].N * Ny;
-    const long long L0 = (long long)tile * 2 * LZ;
-    const int nreal = (int)min((long long)(2 * LZ), nlines - L0);
+    const long long L0 = (long long)tile * 2 * nl;
+    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     const int npair = (nreal + 1) >> 1;
     load_twiddles(tw, az);
     // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) on the rest
-    for (int e = threadIdx.x; e < npair * H; e += blockDim.x) {
-        const int q = e / H;
-        const int k = e - q * H;
-        const long long La = L0 + 2 * q;
-        const long long xa = La / Ny, ya = La - xa * Ny;
-        const float2 A = v.work[(xa * Py + ya) * H + k];
-        float2 B = make_float2(0.f, 0.f);
-        if (2 * q + 1 < nreal) {
-            const long long Lb = La + 1;
-            const long long xb = Lb / Ny, yb = Lb - xb * Ny;
-            B = v.work[(xb * Py + yb) * H + k];
+    const float inv_h = 1.0f / (float)H;
+    const int total = npair * H;
+    for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
+        float2 A[8], B[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            A[u] = make_float2(0.f, 0.f);
+            B[u] = make_float2(0.f, 0.f);
+            if (e < total) {
+                const int q = fast_div(e, H, inv_h);
+                const int k = e - q * H;
+                const long long La = L0 + 2 * q;
+                const long long xa = La / Ny, ya = La - xa * Ny;
+                A[u] = v.work[(xa * Py + ya) * Hp + k];
+                if (2 * q + 1 < nreal) {
+                    const long long Lb = La + 1;
+                    const long long xb = Lb / Ny, yb = Lb - xb * Ny;
+                    B[u] = v.work[(xb * Py + yb) * Hp + k];
+                }
+            }
         }
-        s[k * ld + q] = make_float2(A.x - B.y, A.y + B.x);
-        if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A.x + B.y, B.x - A.y);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * blockDim.x;
+            if (e < total) {
+                const int q = fast_div(e, H, inv_h);
+                const int k = e - q * H;
+                s[k * ld + q] = make_float2(A[u].x - B[u].y, A[u].y + B[u].x);
+                if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
+            }
+        }
     }
-    for (int e = threadIdx.x; e < Pz * (LZ - npair); e += blockDim.x) {   // unused pairs
-        const int k = e / (LZ - npair);
-        const int q = npair + (e - k * (LZ - npair));
-        s[k * ld + q] = make_float2(0.f, 0.f);
+    if (npair < nl) {
+        const int unused = nl - npair;
+        for (int e = threadIdx.x; e < Pz * unused; e += blockDim.x) {
+            const int k = e / unused;
+            s[k * ld + npair + (e - k * unused)] = make_float2(0.f, 0.f);
+        }
     }
     __syncthreads();
-    fft_lines<true>(s, LZ, ld, az, tw);
+    fft_lines<true>(s, lnl, ld, az, tw);
     const long long obase = L0 * Nz;
-    const int total = nreal * Nz;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int ln = e / Nz;
-        const int o = e - ln * Nz;
-        const int q = ln >> 1;
-        const float *p0 = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q]);
-        float val = p0[ln & 1];
-        if (o < az.foldn) {
-            const float *p1 = reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q]);
-            val += p1[ln & 1];
+    const int nout = nreal * Nz;
+    const float inv_nz = 1.0f / (float)Nz;
+    const bool has_dens = v.dens != nullptr;
+    if (v.vec_io) {
+        const int n4 = nout >> 2;
+        for (int i = threadIdx.x; i < n4; i += blockDim.x) {
+            float4 d = make_float4(1.f, 1.f, 1.f, 1.f);
+            if (has_dens) d = reinterpret_cast<const float4 *>(v.dens + obase)[i];
+            float r[4];
+            int e = 4 * i;
+            int ln = fast_div(e, Nz, inv_nz);
+            int o = e - ln * Nz;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int q = ln >> 1;
+                float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
+                if (o < az.foldn)
+                    val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
+                r[c] = val;
+                if (++o == Nz) { o = 0; ++ln; }
+            }
+            const float sc = v.scale;
+            float4 res;
+            if (has_dens) {
+                res = make_float4(r[0] * sc * density_factor(d.x), r[1] * sc * density_factor(d.y),
+                                  r[2] * sc * density_factor(d.z), r[3] * sc * density_factor(d.w));
+            } else {
+                res = make_float4(r[0] * sc, r[1] * sc, r[2] * sc, r[3] * sc);
+            }
+            reinterpret_cast<float4 *>(v.out + obase)[i] = res;
         }
-        float f = v.scale;
-        if (v.dens) f *= density_factor(v.dens[obase + e]);
-        v.out[obase + e] = val * f;
+        for (int e = 4 * n4 + threadIdx.x; e < nout; e += blockDim.x) {
+            const int ln = fast_div(e, Nz, inv_nz);
+            const int o = e - ln * Nz;
+            const int q = ln >> 1;
+            float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
+            if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
+            float f = v.scale;
+            if (has_dens) f *= density_factor(v.dens[obase + e]);
+            v.out[obase + e] = val * f;
+        }
+    } else {
+        for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+            const int ln = fast_div(e, Nz, inv_nz);
+            const int o = e - ln * Nz;
+            const int q = ln >> 1;
+            float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
+            if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
+            float f = v.scale;
+            if (has_dens) f *= density_factor(v.dens[obase + e]);
+            v.out[obase + e] = val * f;
+        }
     }
 }
 

@@ -33,19 +33,20 @@ enum { PVD_PASS_FZ = 0, PVD_PASS_FY = 1, PVD_PASS_X = 2, PVD_PASS_IY = 3, PVD_PA
 struct pvd_vol_d {
     pvd_axis_d ax[3];                 // x, y, z
     int H;                            // Pz/2 + 1 (half spectrum along z)
+    int Hp;                           // row pitch of work/spec: H rounded up to 16 (128-byte rows)
     int T;                            // timepoints folded into the load
-    int lz;                           // complex line pairs per z-pass tile
-    int tky;                          // columns per y-pass tile
-    int tkx;                          // columns per x-pass tile
-    int pad_;
+    int llz;                          // log2 of complex line pairs per z-pass tile
+    int ltky;                         // log2 of columns per y-pass tile
+    int ltkx;                         // log2 of columns per x-pass tile
+    int vec_io;                       // 1: act/dens/out are 16-byte aligned (float4 access)
     const float *act[PVD_MAX_TIMEPOINTS];
     float w[PVD_MAX_TIMEPOINTS];
     const float *dens;
     float *out;
     float scale;
     int pad2_;
-    float2 *work;                     // [Nx][Py][H] complex workspace
-    const float2 *spec;               // [Px][Py][H] kernel spectrum (1/P folded in)
+    float2 *work;                     // [Nx][Py][Hp] complex workspace
+    const float2 *spec;               // [Px][Py][Hp] kernel spectrum (1/P folded in)
     int tile_base[PVD_NPASS_FFT];     // first grid tile of this volume per pass (batch)
     int tiles[PVD_NPASS_FFT];         // tiles of this volume per pass
 };
@@ -198,7 +199,17 @@ template <bool INV> struct Dft<7, INV> {
 };
 
 // ---------------------------------------------------------------------------------
-// One Stockham stage of radix R over `nl` lines held in LDS, element
+// a / d for 0 <= a < 2^22 and 1 <= d <= 2^16 from a float reciprocal plus one
+// correction step (the float estimate is within 1 of the true quotient).
+__device__ __forceinline__ int fast_div(int a, int d, float inv_d) {
+    int q = (int)((float)a * inv_d);
+    const int r = a - q * d;
+    if (r >= d) ++q;
+    else if (r < 0) --q;
+    return q;
+}
+
+// One Stockham stage of radix R over nl = 2^lnl lines held in LDS, element
 // (line l, index k) at s[k*ld + l].  Ns = product of the radices of the stages
 // already done.  Every thread keeps up to IT butterflies in registers between
 // the read and the write phase so the stage is in place; the launch guarantees
@@ -206,12 +217,13 @@ template <bool INV> struct Dft<7, INV> {
 template <int R> struct StageItems { static constexpr int v = (16 + R - 1) / R; };
 
 template <int R, bool INV>
-__device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int nl, int ld, int n, int Ns,
+__device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int lnl, int ld, int n, int Ns,
                                                const float2 *__restrict__ tw) {
     constexpr int IT = StageItems<R>::v;
     const int m = n / R;
-    const int work = nl * m;
+    const int work = m << lnl;
     const int tstride = n / (Ns * R);
+    const float inv_ns = 1.0f / (float)Ns;
     float2 v[IT][R];
     int dst[IT];
 #pragma unroll
@@ -219,9 +231,10 @@ __device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int nl, i
         const int w = threadIdx.x + it * blockDim.x;
         dst[it] = -1;
         if (w < work) {
-            const int j = w / nl;
-            const int l = w - j * nl;
-            const int k = j % Ns;
+            const int j = w >> lnl;
+            const int l = w & ((1 << lnl) - 1);
+            const int q = fast_div(j, Ns, inv_ns);
+            const int k = j - q * Ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[it][r] = s[(j + r * m) * ld + l];
             if (k != 0) {
@@ -234,7 +247,7 @@ __device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int nl, i
                 }
             }
             Dft<R, INV>::run(v[it]);
-            dst[it] = ((j - k) * R + k) * ld + l;
+            dst[it] = (q * Ns * R + k) * ld + l;
         }
     }
     __syncthreads();
@@ -248,23 +261,23 @@ __device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int nl, i
     __syncthreads();
 }
 
-// Full FFT of length ax.n over nl lines in LDS (natural order in and out,
+// Full FFT of length ax.n over 2^lnl lines in LDS (natural order in and out,
 // unnormalised).  Must be entered after a barrier that published `s` and `tw`.
 template <bool INV>
-__device__ __forceinline__ void fft_lines(float2 *__restrict__ s, int nl, int ld, const pvd_axis_d &ax,
+__device__ __forceinline__ void fft_lines(float2 *__restrict__ s, int lnl, int ld, const pvd_axis_d &ax,
                                           const float2 *__restrict__ tw) {
     const int n = ax.n;
     int Ns = 1;
     for (int st = 0; st < ax.nst; ++st) {
         const int R = ax.radix[st];
         switch (R) {
-            case 16: stockham_stage<16, INV>(s, nl, ld, n, Ns, tw); break;
-            case 8: stockham_stage<8, INV>(s, nl, ld, n, Ns, tw); break;
-            case 4: stockham_stage<4, INV>(s, nl, ld, n, Ns, tw); break;
-            case 2: stockham_stage<2, INV>(s, nl, ld, n, Ns, tw); break;
-            case 3: stockham_stage<3, INV>(s, nl, ld, n, Ns, tw); break;
-            case 5: stockham_stage<5, INV>(s, nl, ld, n, Ns, tw); break;
-            case 7: stockham_stage<7, INV>(s, nl, ld, n, Ns, tw); break;
+            case 16: stockham_stage<16, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 8: stockham_stage<8, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 4: stockham_stage<4, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 2: stockham_stage<2, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 3: stockham_stage<3, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 5: stockham_stage<5, INV>(s, lnl, ld, n, Ns, tw); break;
+            case 7: stockham_stage<7, INV>(s, lnl, ld, n, Ns, tw); break;
             default: break;
         }
         Ns *= R;

@@ -70,6 +70,7 @@ struct pvd_spec_args {
     pvd_spec_axis a[3];
     int K[3];
     int H;
+    int Hp;   // row pitch of the FP32 spectrum
 };
 
 __device__ __forceinline__ int spec_pos(const pvd_spec_axis &ax, int m) {
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(256) void pvd_spec_y(pvd_spec_args p, const double2
     }
 }
 
-// S[kx][ky][kz] (FP32) = norm * sum_a T2[a][ky][kz] W_Px^{kx pos_x(a)}
+// S[kx][ky][kz] (FP32, rows of pitch Hp) = norm * sum_a T2[a][ky][kz] W_Px^{kx pos_x(a)}
 __global__ __launch_bounds__(256) void pvd_spec_x(pvd_spec_args p, const double2 *__restrict__ T2,
                                                   double norm, float2 *__restrict__ S) {
     extern __shared__ double2 tw64[];
@@ -160,7 +161,8 @@ __global__ __launch_bounds__(256) void pvd_spec_x(pvd_spec_args p, const double2
             acc.x += r.x;
             acc.y += r.y;
         }
-        S[idx] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
+        const long long ky = c / p.H, kz = c - ky * p.H;
+        S[((long long)kx * p.a[1].P + ky) * p.Hp + kz] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
     }
 }
 
@@ -172,13 +174,14 @@ static int grid_for(long long total, int block) {
 
 // T1/T2 are caller-provided FP64 scratch of ux*uy*H and ux*Py*H double2.
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
-                               const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st) {
+                               int Hp, const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st) {
     pvd_spec_args p;
     for (int i = 0; i < 3; ++i) {
         p.a[i].P = P[i]; p.a[i].lo = lo[i]; p.a[i].u = u[i]; p.a[i].s = s[i];
         p.K[i] = K[i];
     }
     p.H = P[2] / 2 + 1;
+    p.Hp = Hp;
     const int block = 256;
     hipLaunchKernelGGL(pvd_spec_z, dim3(grid_for((long long)u[0] * u[1] * p.H, block)), dim3(block),
                        sizeof(double2) * P[2], st, p, kernel, T1);

@@ -239,7 +239,9 @@ def test_plan_rejects_bad_inputs():
     with pytest.raises(ValueError):
         ConvPlan((8, 8, 8), (3, 3, 3), fft_n=(11, 8, 8))       # not 7-smooth
     with pytest.raises(NotImplementedError):
-        ConvPlan((8, 8, 8), (31, 3, 3), mode="linear", algo="direct")
+        ConvPlan((40, 8, 8), (31, 3, 3), mode="linear", algo="direct")     # 31 taps used: > 16
+    # a 31-wide kernel on an 8-wide grid only ever uses 15 taps in linear mode
+    assert ConvPlan((8, 8, 8), (31, 3, 3), mode="linear", algo="direct").kernel_used == (15, 3, 3)
     plan = ConvPlan((8, 8, 8), (3, 3, 3))
     a = torch.zeros((8, 8, 8), device="cuda")
     with pytest.raises(ValueError, match="no kernel"):
