@@ -134,6 +134,22 @@ int pvd_conv_dose_timeseries(pvd_plan *plan, const float *const *acts,
 int pvd_weighted_sum(const float *const *maps, const double *weights, int T, size_t n,
                      float *out, void *stream);
 
+/* Batches: many volumes of uneven sizes (each with its own FFT plan and kernel)
+ * go through ONE launch per FFT pass; every launch's grid covers the tiles of
+ * all volumes and each workgroup reads its volume's descriptor from a device
+ * table.  pvd_batch_run_graph captures the five launches once into a hipGraph
+ * and replays it; the descriptor table is uploaded before each replay, so new
+ * pointers/weights take effect without re-capture.  No counterpart in the
+ * reference, whose calculate_absorbed_dose (kernel_convolution.py:78-106) is
+ * called once per volume. */
+int pvd_batch_create(pvd_batch **batch, pvd_plan *const *plans, int nvol);
+/* Set volume i's inputs: dose_i = scale * (rho_w/rho) * conv(sum_t w_t acts_t). */
+int pvd_batch_set_volume(pvd_batch *batch, int i, const float *const *acts, const double *weights, int T,
+                         const float *density, float *dose, double scale);
+int pvd_batch_run(pvd_batch *batch, void *stream);
+int pvd_batch_run_graph(pvd_batch *batch, void *stream);
+int pvd_batch_destroy(pvd_batch *batch);
+
 /* Per-pass GPU milliseconds of the last dose call of a PVD_FLAG_PROFILE plan
  * (synchronises on it).  Writes min(n, n_passes) values; returns n_passes. */
 int pvd_plan_timings(pvd_plan *plan, float *ms, int n);

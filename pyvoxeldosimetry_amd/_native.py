@@ -66,6 +66,11 @@ SIGNATURES = {
     "pvd_conv_dose_weighted": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D), _I, _P, _P, _D, _P]),
     "pvd_conv_dose_timeseries": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_D), _I, _P, _P, _D, _P]),
     "pvd_weighted_sum": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_D), _I, ctypes.c_size_t, _P, _P]),
+    "pvd_batch_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(_P), _I]),
+    "pvd_batch_set_volume": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_D), _I, _P, _P, _D]),
+    "pvd_batch_run": (_I, [_P, _P]),
+    "pvd_batch_run_graph": (_I, [_P, _P]),
+    "pvd_batch_destroy": (_I, [_P]),
     "pvd_plan_timings": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "pvd_pass_name": (ctypes.c_char_p, [_P, _I]),
 }

@@ -166,6 +166,16 @@ struct pvd_plan {
 
 struct pvd_batch {
     std::vector<pvd_plan *> plans;
+    int dev = 0;
+    int nvol = 0;
+    pvd_vol_d *h_tab = nullptr;        // pinned host copy of the descriptor table
+    pvd_vol_d *d_tab = nullptr;        // device table read by the batch kernels
+    std::vector<char> ready;           // volume inputs set
+    launch_cfg cfg[PVD_NPASS_FFT];
+    hipEvent_t uploaded = nullptr;     // last table upload (host table reusable after it)
+    bool upload_pending = false;
+    hipStream_t cap = nullptr;         // private stream used only for graph capture
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -303,8 +313,26 @@ int launch_fft(pvd_plan *p, const pvd_vol_d &v, hipStream_t st) {
     return PVD_OK;
 }
 
-int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const float *dens, float *out,
-             double scale, hipStream_t st) {
+// the plan's descriptor template completed with one call's inputs
+void fill_desc(const pvd_plan *p, const float *const *acts, const float *w, int T, const float *dens, float *out,
+               double scale, pvd_vol_d *vd) {
+    pvd_vol_d &v = *vd;
+    v = p->vol;
+    v.T = T;
+    for (int t = 0; t < PVD_MAX_TIMEPOINTS; ++t) {
+        v.act[t] = t < T ? acts[t] : nullptr;
+        v.w[t] = t < T ? w[t] : 0.f;
+    }
+    v.dens = dens;
+    v.out = out;
+    v.scale = (float)scale;
+    // float4 access when every z-line starts 16-byte aligned
+    bool vec = p->N[2] % 4 == 0 && aligned16(out) && aligned16(dens);
+    for (int t = 0; t < T; ++t) vec = vec && aligned16(acts[t]);
+    v.vec_io = vec ? 1 : 0;
+}
+
+int check_inputs(const pvd_plan *p, const float *const *acts, int T, const float *out, double scale) {
     if (!p) return fail(PVD_EINVAL, "null plan");
     if (!p->has_kernel) return fail(PVD_EINVAL, "plan has no kernel: call pvd_plan_set_kernel first");
     if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
@@ -312,6 +340,12 @@ int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const
     for (int t = 0; t < T; ++t)
         if (!acts[t]) return fail(PVD_EINVAL, "null activity pointer %d", t);
     if (!isfinite(scale)) return fail(PVD_EINVAL, "non-finite scale");
+    return PVD_OK;
+}
+
+int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const float *dens, float *out,
+             double scale, hipStream_t st) {
+    if (int rc = check_inputs(p, acts, T, out, scale)) return rc;
     PVD_HIP(hipSetDevice(p->dev), "set device");
     if (p->algo == PVD_ALGO_DIRECT) {
         const bool prof = (p->prm.flags & PVD_FLAG_PROFILE) != 0;
@@ -326,19 +360,8 @@ int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const
         p->profiled_call = prof;
         return PVD_OK;
     }
-    pvd_vol_d v = p->vol;
-    v.T = T;
-    for (int t = 0; t < PVD_MAX_TIMEPOINTS; ++t) {
-        v.act[t] = t < T ? acts[t] : nullptr;
-        v.w[t] = t < T ? w[t] : 0.f;
-    }
-    v.dens = dens;
-    v.out = out;
-    v.scale = (float)scale;
-    // float4 access when every tile's first voxel is 16-byte aligned
-    bool vec = ((2LL << v.llz) * p->N[2]) % 4 == 0 && aligned16(out) && aligned16(dens);
-    for (int t = 0; t < T; ++t) vec = vec && aligned16(acts[t]);
-    v.vec_io = vec ? 1 : 0;
+    pvd_vol_d v;
+    fill_desc(p, acts, w, T, dens, out, scale, &v);
     return launch_fft(p, v, st);
 }
 
@@ -532,6 +555,170 @@ int pvd_plan_timings(pvd_plan *p, float *ms, int n) {
     for (int i = 0; i < np && i < n; ++i)
         PVD_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]), "event time");
     return np;
+}
+
+// ---------------------------------------------------------------------------------
+// batches
+int pvd_batch_destroy(pvd_batch *b) {
+    if (!b) return PVD_OK;
+    if (b->uploaded) (void)hipEventSynchronize(b->uploaded);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->h_tab) (void)hipHostFree(b->h_tab);
+    delete b;
+    return PVD_OK;
+}
+
+int pvd_batch_create(pvd_batch **out, pvd_plan *const *plans, int nvol) {
+    if (!out || !plans || nvol < 1) return fail(PVD_EINVAL, "need at least one plan");
+    *out = nullptr;
+    for (int i = 0; i < nvol; ++i) {
+        if (!plans[i]) return fail(PVD_EINVAL, "null plan %d", i);
+        if (plans[i]->algo != PVD_ALGO_FFT) return fail(PVD_EINVAL, "plan %d is not an FFT plan", i);
+        if (!plans[i]->has_kernel) return fail(PVD_EINVAL, "plan %d has no kernel", i);
+        if (plans[i]->dev != plans[0]->dev) return fail(PVD_EINVAL, "plans on different devices");
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i]) return fail(PVD_EINVAL, "plan %d repeated (volumes need their own workspace)", i);
+    }
+    pvd_batch *b = new (std::nothrow) pvd_batch();
+    if (!b) return fail(PVD_ENOMEM, "host allocation");
+    b->plans.assign(plans, plans + nvol);
+    b->nvol = nvol;
+    b->dev = plans[0]->dev;
+    b->ready.assign(nvol, 0);
+    int rc = PVD_OK;
+    do {
+        if (hipSetDevice(b->dev) != hipSuccess) { rc = fail(PVD_EHIP, "set device"); break; }
+        if (hipHostMalloc((void **)&b->h_tab, sizeof(pvd_vol_d) * nvol, hipHostMallocDefault) != hipSuccess) {
+            rc = fail(PVD_ENOMEM, "pinned descriptor table");
+            break;
+        }
+        if (hipMalloc(&b->d_tab, sizeof(pvd_vol_d) * nvol) != hipSuccess) {
+            rc = fail(PVD_ENOMEM, "device descriptor table");
+            break;
+        }
+        if (hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming) != hipSuccess) {
+            rc = fail(PVD_EHIP, "create event");
+            break;
+        }
+        for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
+            long long total = 0;
+            launch_cfg c;
+            for (int i = 0; i < nvol; ++i) {
+                const launch_cfg &pc = plans[i]->cfg[pass];
+                c.block = std::max(c.block, pc.block);
+                c.lds = std::max(c.lds, pc.lds);
+                total += pc.grid;
+            }
+            if (total > 0x7fffffffLL) { rc = fail(PVD_ENOTSUP, "batch grid too large"); break; }
+            c.grid = (int)total;
+            b->cfg[pass] = c;
+        }
+        if (rc) break;
+        for (int i = 0; i < nvol; ++i) {
+            b->h_tab[i] = plans[i]->vol;
+            for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
+                int base = 0;
+                for (int j = 0; j < i; ++j) base += plans[j]->cfg[pass].grid;
+                b->h_tab[i].tile_base[pass] = base;
+                b->h_tab[i].tiles[pass] = plans[i]->cfg[pass].grid;
+            }
+        }
+    } while (0);
+    if (rc) {
+        pvd_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PVD_OK;
+}
+
+int pvd_batch_set_volume(pvd_batch *b, int i, const float *const *acts, const double *weights, int T,
+                         const float *density, float *dose, double scale) {
+    if (!b) return fail(PVD_EINVAL, "null batch");
+    if (i < 0 || i >= b->nvol) return fail(PVD_EINVAL, "volume index %d outside [0, %d)", i, b->nvol);
+    if (!acts || !weights) return fail(PVD_EINVAL, "null argument");
+    if (T < 1 || T > PVD_MAX_TIMEPOINTS) return fail(PVD_EINVAL, "T=%d outside [1, %d]", T, PVD_MAX_TIMEPOINTS);
+    float w[PVD_MAX_TIMEPOINTS];
+    for (int t = 0; t < T; ++t) {
+        if (!isfinite(weights[t])) return fail(PVD_EINVAL, "non-finite weight %d", t);
+        w[t] = (float)weights[t];
+    }
+    pvd_plan *p = b->plans[i];
+    if (int rc = check_inputs(p, acts, T, dose, scale)) return rc;
+    if (b->upload_pending) {   // the previous upload still reads the host table
+        PVD_HIP(hipEventSynchronize(b->uploaded), "wait for descriptor upload");
+        b->upload_pending = false;
+    }
+    pvd_vol_d v;
+    fill_desc(p, acts, w, T, density, dose, scale, &v);
+    for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
+        v.tile_base[pass] = b->h_tab[i].tile_base[pass];
+        v.tiles[pass] = b->h_tab[i].tiles[pass];
+    }
+    b->h_tab[i] = v;
+    b->ready[i] = 1;
+    return PVD_OK;
+}
+
+namespace {
+int batch_upload(pvd_batch *b, hipStream_t st) {
+    for (int i = 0; i < b->nvol; ++i)
+        if (!b->ready[i]) return fail(PVD_EINVAL, "volume %d has no inputs (pvd_batch_set_volume)", i);
+    PVD_HIP(hipSetDevice(b->dev), "set device");
+    PVD_HIP(hipMemcpyAsync(b->d_tab, b->h_tab, sizeof(pvd_vol_d) * b->nvol, hipMemcpyHostToDevice, st),
+            "upload descriptor table");
+    PVD_HIP(hipEventRecord(b->uploaded, st), "record event");
+    b->upload_pending = true;
+    return PVD_OK;
+}
+
+int batch_launch(pvd_batch *b, hipStream_t st) {
+    pvd_batch_args a;
+    a.vols = b->d_tab;
+    a.nvol = b->nvol;
+    for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
+        const launch_cfg &c = b->cfg[pass];
+        PVD_HIP(pvd_launch_fft_pass_batch(pass, a, c.grid, c.block, c.lds, st), "launch batch FFT pass");
+    }
+    return PVD_OK;
+}
+}  // namespace
+
+int pvd_batch_run(pvd_batch *b, void *stream) {
+    if (!b) return fail(PVD_EINVAL, "null batch");
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = batch_upload(b, st)) return rc;
+    return batch_launch(b, st);
+}
+
+int pvd_batch_run_graph(pvd_batch *b, void *stream) {
+    if (!b) return fail(PVD_EINVAL, "null batch");
+    hipStream_t st = (hipStream_t)stream;
+    if (!b->exec) {
+        PVD_HIP(hipSetDevice(b->dev), "set device");
+        if (!b->cap) PVD_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "create capture stream");
+        PVD_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "begin capture");
+        int rc = batch_launch(b, b->cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "end capture");
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->exec = nullptr;
+            return hip_fail(e, "instantiate graph");
+        }
+    }
+    if (int rc = batch_upload(b, st)) return rc;
+    PVD_HIP(hipGraphLaunch(b->exec, st), "launch graph");
+    return PVD_OK;
 }
 
 const char *pvd_pass_name(const pvd_plan *p, int i) {

@@ -50,9 +50,35 @@ __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const floa
 }
 
 // ---------------------------------------------------------------------------------
+// Offsets in the workspace of the (up to 32) real z-lines of a z-pass tile.
+__device__ __forceinline__ void tile_line_offsets(long long *lineoff, long long L0, int nreal, int Ny, int Py,
+                                                  int Hp) {
+    if (threadIdx.x < nreal) {
+        const int L = (int)(L0 + threadIdx.x);
+        const int x = fast_div(L, Ny, 1.0f / (float)Ny);
+        lineoff[threadIdx.x] = ((long long)x * Py + (L - x * Ny)) * Hp;
+    }
+}
+
+// float4 lane mapping shared by the z passes: 16 consecutive lanes cover 16
+// consecutive float4 of one line, neighbouring 16-lane groups take the next
+// lines, so a wavefront's LDS scatter/gather touches 4 line columns at once.
+struct ZVec {
+    int items, nreal16, nf4;
+    __device__ ZVec(int nreal, int Nz) : nreal16(nreal * 16), nf4(Nz >> 2) { items = ((nf4 + 15) >> 4) * nreal16; }
+    __device__ __forceinline__ bool decode(int i, float inv, int &ln, int &f) const {
+        const int chunk = fast_div(i, nreal16, inv);
+        const int rem = i - chunk * nreal16;
+        ln = rem >> 4;
+        f = (chunk << 4) + (rem & 15);
+        return i < items && f < nf4;
+    }
+};
+
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
 // holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1).
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    __shared__ long long lineoff[32];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
@@ -61,6 +87,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const long long L0 = (long long)tile * 2 * nl;
     const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     load_twiddles(tw, az);
+    tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
     // zero the padding: rows z >= Nz of every line, and the unused line pairs
     for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
     const int npair = (nreal + 1) >> 1;
@@ -71,29 +98,32 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         }
         __syncthreads();
     }
-    // weighted sum of the T maps over one contiguous chunk of nreal*Nz floats
+    // weighted sum of the T maps over the tile's nreal*Nz contiguous floats
     const long long base = L0 * Nz;
-    const int total = nreal * Nz;
-    const float inv_nz = 1.0f / (float)Nz;
-    if (v.vec_io) {
-        const int n4 = total >> 2;      // base is a multiple of 4 floats (2*nl*Nz*tile)
-        for (int i0 = threadIdx.x; i0 < n4; i0 += 8 * blockDim.x) {
+    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps
+        const ZVec zv(nreal, Nz);
+        const float inv = 1.0f / (float)zv.nreal16;
+        for (int i0 = threadIdx.x; i0 < zv.items; i0 += 8 * blockDim.x) {
             float4 a[8];
+            int lnv[8], fv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int i = i0 + u * blockDim.x;
+                int ln, f;
+                const bool ok = zv.decode(i0 + u * blockDim.x, inv, ln, f);
+                lnv[u] = ok ? ln : -1;
+                fv[u] = f;
                 a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (i < n4) {
-                    const float4 x = reinterpret_cast<const float4 *>(v.act[0] + base)[i];
+                if (ok) {
+                    const float4 x = reinterpret_cast<const float4 *>(v.act[0] + base + (long long)ln * Nz)[f];
                     a[u] = make_float4(v.w[0] * x.x, v.w[0] * x.y, v.w[0] * x.z, v.w[0] * x.w);
                 }
             }
             for (int t = 1; t < v.T; ++t) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const int i = i0 + u * blockDim.x;
-                    if (i < n4) {
-                        const float4 x = reinterpret_cast<const float4 *>(v.act[t] + base)[i];
+                    if (lnv[u] >= 0) {
+                        const float4 x =
+                            reinterpret_cast<const float4 *>(v.act[t] + base + (long long)lnv[u] * Nz)[fv[u]];
                         a[u].x = fmaf(v.w[t], x.x, a[u].x);
                         a[u].y = fmaf(v.w[t], x.y, a[u].y);
                         a[u].z = fmaf(v.w[t], x.z, a[u].z);
@@ -103,27 +133,18 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int i = i0 + u * blockDim.x;
-                if (i < n4) {
-                    const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
-                    int e = 4 * i;
-                    int ln = fast_div(e, Nz, inv_nz);
-                    int z = e - ln * Nz;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        reinterpret_cast<float *>(&s[z * ld + (ln >> 1)])[ln & 1] = av[c];
-                        if (++z == Nz) { z = 0; ++ln; }
-                    }
+                if (lnv[u] >= 0) {
+                    float *col = reinterpret_cast<float *>(&s[(4 * fv[u]) * ld + (lnv[u] >> 1)]) + (lnv[u] & 1);
+                    col[0] = a[u].x;
+                    col[2 * ld] = a[u].y;
+                    col[4 * ld] = a[u].z;
+                    col[6 * ld] = a[u].w;
                 }
             }
         }
-        for (int e = 4 * n4 + threadIdx.x; e < total; e += blockDim.x) {   // tail
-            float a = v.w[0] * v.act[0][base + e];
-            for (int t = 1; t < v.T; ++t) a = fmaf(v.w[t], v.act[t][base + e], a);
-            const int ln = fast_div(e, Nz, inv_nz);
-            reinterpret_cast<float *>(&s[(e - ln * Nz) * ld + (ln >> 1)])[ln & 1] = a;
-        }
     } else {
+        const int total = nreal * Nz;
+        const float inv_nz = 1.0f / (float)Nz;
         for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
             float a[8];
 #pragma unroll
@@ -160,9 +181,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         const float2 zm = s[(k == 0 ? 0 : Pz - k) * ld + q];
         const float2 r = (ln & 1) ? make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x))
                                   : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        const long long L = L0 + ln;
-        const long long x = L / Ny, y = L - x * Ny;
-        v.work[(x * Py + y) * Hp + k] = r;
+        v.work[lineoff[ln] + k] = r;
     }
 }
 
@@ -247,7 +266,15 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 
 // ---------------------------------------------------------------------------------
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
+__device__ __forceinline__ float iz_value(const float2 *s, const pvd_axis_d &az, int ld, int o, int ln) {
+    const int q = ln >> 1;
+    float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, az.n) * ld + q])[ln & 1];
+    if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + az.N + az.off) % az.n) * ld + q])[ln & 1];
+    return val;
+}
+
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    __shared__ long long lineoff[32];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
@@ -257,35 +284,31 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     const int npair = (nreal + 1) >> 1;
     load_twiddles(tw, az);
+    tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
+    __syncthreads();
     // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) on the rest
     const float inv_h = 1.0f / (float)H;
     const int total = npair * H;
     for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
         float2 A[8], B[8];
+        int qv[8], kv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * blockDim.x;
+            const int q = fast_div(e, H, inv_h);
+            qv[u] = e < total ? q : -1;
+            kv[u] = e - q * H;
             A[u] = make_float2(0.f, 0.f);
             B[u] = make_float2(0.f, 0.f);
             if (e < total) {
-                const int q = fast_div(e, H, inv_h);
-                const int k = e - q * H;
-                const long long La = L0 + 2 * q;
-                const long long xa = La / Ny, ya = La - xa * Ny;
-                A[u] = v.work[(xa * Py + ya) * Hp + k];
-                if (2 * q + 1 < nreal) {
-                    const long long Lb = La + 1;
-                    const long long xb = Lb / Ny, yb = Lb - xb * Ny;
-                    B[u] = v.work[(xb * Py + yb) * Hp + k];
-                }
+                A[u] = v.work[lineoff[2 * q] + kv[u]];
+                if (2 * q + 1 < nreal) B[u] = v.work[lineoff[2 * q + 1] + kv[u]];
             }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < total) {
-                const int q = fast_div(e, H, inv_h);
-                const int k = e - q * H;
+            if (qv[u] >= 0) {
+                const int q = qv[u], k = kv[u];
                 s[k * ld + q] = make_float2(A[u].x - B[u].y, A[u].y + B[u].x);
                 if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
             }
@@ -301,55 +324,35 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __syncthreads();
     fft_lines<true>(s, lnl, ld, az, tw);
     const long long obase = L0 * Nz;
-    const int nout = nreal * Nz;
-    const float inv_nz = 1.0f / (float)Nz;
     const bool has_dens = v.dens != nullptr;
-    if (v.vec_io) {
-        const int n4 = nout >> 2;
-        for (int i = threadIdx.x; i < n4; i += blockDim.x) {
+    const float sc = v.scale;
+    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps
+        const ZVec zv(nreal, Nz);
+        const float inv = 1.0f / (float)zv.nreal16;
+        for (int i = threadIdx.x; i < zv.items; i += blockDim.x) {
+            int ln, f;
+            if (!zv.decode(i, inv, ln, f)) continue;
+            const long long off = obase + (long long)ln * Nz;
             float4 d = make_float4(1.f, 1.f, 1.f, 1.f);
-            if (has_dens) d = reinterpret_cast<const float4 *>(v.dens + obase)[i];
-            float r[4];
-            int e = 4 * i;
-            int ln = fast_div(e, Nz, inv_nz);
-            int o = e - ln * Nz;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int q = ln >> 1;
-                float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
-                if (o < az.foldn)
-                    val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
-                r[c] = val;
-                if (++o == Nz) { o = 0; ++ln; }
-            }
-            const float sc = v.scale;
-            float4 res;
+            if (has_dens) d = reinterpret_cast<const float4 *>(v.dens + off)[f];
+            const int o = 4 * f;
+            float4 res = make_float4(iz_value(s, az, ld, o, ln) * sc, iz_value(s, az, ld, o + 1, ln) * sc,
+                                     iz_value(s, az, ld, o + 2, ln) * sc, iz_value(s, az, ld, o + 3, ln) * sc);
             if (has_dens) {
-                res = make_float4(r[0] * sc * density_factor(d.x), r[1] * sc * density_factor(d.y),
-                                  r[2] * sc * density_factor(d.z), r[3] * sc * density_factor(d.w));
-            } else {
-                res = make_float4(r[0] * sc, r[1] * sc, r[2] * sc, r[3] * sc);
+                res.x *= density_factor(d.x);
+                res.y *= density_factor(d.y);
+                res.z *= density_factor(d.z);
+                res.w *= density_factor(d.w);
             }
-            reinterpret_cast<float4 *>(v.out + obase)[i] = res;
-        }
-        for (int e = 4 * n4 + threadIdx.x; e < nout; e += blockDim.x) {
-            const int ln = fast_div(e, Nz, inv_nz);
-            const int o = e - ln * Nz;
-            const int q = ln >> 1;
-            float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
-            if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
-            float f = v.scale;
-            if (has_dens) f *= density_factor(v.dens[obase + e]);
-            v.out[obase + e] = val * f;
+            reinterpret_cast<float4 *>(v.out + off)[f] = res;
         }
     } else {
+        const int nout = nreal * Nz;
+        const float inv_nz = 1.0f / (float)Nz;
         for (int e = threadIdx.x; e < nout; e += blockDim.x) {
             const int ln = fast_div(e, Nz, inv_nz);
-            const int o = e - ln * Nz;
-            const int q = ln >> 1;
-            float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, Pz) * ld + q])[ln & 1];
-            if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + Nz + az.off) % Pz) * ld + q])[ln & 1];
-            float f = v.scale;
+            const float val = iz_value(s, az, ld, e - ln * Nz, ln);
+            float f = sc;
             if (has_dens) f *= density_factor(v.dens[obase + e]);
             v.out[obase + e] = val * f;
         }

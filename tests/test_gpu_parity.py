@@ -251,3 +251,21 @@ def test_plan_rejects_bad_inputs():
         plan.dose(torch.zeros((8, 8, 9), device="cuda"))
     with pytest.raises(ValueError):
         plan.dose(a.double())
+
+
+def test_torso_config_512x512x400_61cubed_6_timepoints():
+    """Full-resolution volume: linear (padded) FFT, Lu-177 61^3, T=6, density; FP64 oracle."""
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    from pyvoxeldosimetry_amd.phantoms import torso_phantom
+    _say("torso 512x512x400 T=6")
+    ph = torso_phantom()
+    k = Lu177KernelGenerator("water").generate_kernel_device(ph.voxel_mm, ph.kernel_grid)
+    plan = ConvPlan(ph.shape, ph.kernel_grid, mode="linear", algo="fft")
+    plan.set_kernel(k)
+    got = plan.dose_timeseries([dev_tensor(a) for a in ph.activity], ph.time_points_h,
+                               density=dev_tensor(ph.density)).cpu().numpy()
+    w = conv.trapezoid_weights_seconds(ph.time_points_h)
+    acc = sum(wi * a.astype(np.float64) for wi, a in zip(w, ph.activity))
+    exp = conv.epilogue(conv.linear_same(acc, k.cpu().numpy()), ph.density)
+    m = assert_parity(got, exp, what="torso/linear")
+    _say(f"torso max_rel={m['max_rel']:.2e} rms_rel={m['rms_rel']:.2e} fft_n={plan.fft_n}")
