@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gvoxel/s dose conv 256x256x200 (FP64 err≤1e-5); CNN-corrected ms/vol @ ≤2% err"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_fft_x_spectrum_multiply_ct_256x256x200.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fft_passes_ct_256x256x200.md")
 
 
 def pass_bytes(plan, T: int, density: bool):
@@ -54,10 +54,8 @@ def read_traffic(kernel_name: str):
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", text)
-    if m and kernel_name in text:
-        return float(m.group(1))
-    return None
+    m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel_name) + r"\]\s*=\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(act64, kernel64, reps: int):
