@@ -1,0 +1,106 @@
+/* pvd_learned.h -- C ABI of the learned heterogeneity correction in libpvd.so:
+ * the project's Monte Carlo target generator and the BF16 MFMA kernels of the
+ * 3-D residual CNN (forward, backward, optimizer).
+ *
+ * Conventions as in pvd.h: device pointers, a hipStream_t passed as void*,
+ * int return codes (PVD_OK or PVD_E*), messages from pvd_last_error().
+ *
+ * None of these has a working counterpart in the reference: its GPU Monte
+ * Carlo (pyvoxeldosimetry/core/monte_carlo.py:10-173) cannot be constructed
+ * and deposits energy one step from the emission point, and it has no learned
+ * correction.  The Monte Carlo replaces that placeholder as the source of
+ * heterogeneous-medium dose.
+ */
+#ifndef PVD_LEARNED_H
+#define PVD_LEARNED_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- Monte Carlo */
+typedef struct pvd_mc_params {
+    int n[3];                 /* grid (x, y, z), z contiguous                         */
+    double voxel_mm;          /* isotropic voxel size                                 */
+    int histories;            /* electrons per voxel with activity > 0               */
+    unsigned long long seed;
+    const float *e_of_u;      /* device: inverse CDF of the beta spectrum, n_u points */
+    int n_u;
+    const float *e_of_r;      /* device: E(r) on r in [0, r_max_mm], n_r points       */
+    int n_r;
+    double r_max_mm;          /* largest water-equivalent range in the table          */
+    double katz_a, katz_b, katz_c;  /* R(E) = 10*a*E^(b - c ln E) mm in water (E MeV)  */
+    double step_mm;           /* largest water-equivalent step                        */
+    double scatter;           /* multiple-scattering strength                         */
+    double rho_min;           /* densities below are clamped (g/cc)                   */
+    const long long *active;  /* device: flat indices of the emitting voxels, or NULL   */
+    long long n_active;       /*   (NULL: every voxel, those with activity 0 emit none) */
+} pvd_mc_params;
+
+/* edep[v] += energy (MeV x activity weight) deposited in voxel v (FP64: a voxel
+ * may receive ~1e7 deposits of ~1e-8 of its total); totals[0..2] += emitted,
+ * deposited, escaped (FP64).  density may be NULL (water). */
+int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const float *density, double *edep,
+                double *totals, void *stream);
+
+/* ---------------------------------------------------------------- 3-D CNN
+ * Activations are channels-last BF16 [n0][n1][n2][C] with n2 % 32 == 0.
+ * Weights of a 3x3x3 layer are BF16 [27][32][cin] (tap-major, cout, cin
+ * contiguous), tap = (dx+1)*9 + (dy+1)*3 + (dz+1); convolution is the
+ * cross-correlation y[v] = sum_tap,ci x[v + d(tap)][ci] w[tap][co][ci] with zero
+ * padding outside the grid.  cout is always 32; cin is 16 or 32. */
+#define PVD_CNN_EPI_RELU        0  /* a = relu(z + b): writes a (bf16) to y and to act  */
+#define PVD_CNN_EPI_RESID_RELU  1  /* a = relu(z + b): act = a, y = resid + a (bf16)     */
+#define PVD_CNN_EPI_LINEAR_F32  2  /* y32[v] = z[v][0] + b[0]  (FP32, channel 0 only)    */
+#define PVD_CNN_EPI_ACCUM_F32   3  /* y32[v][c] += z[v][c]      (FP32, all 32 channels)   */
+
+int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, int epi, const void *resid,
+                     void *y, void *act, float *y32, int n0, int n1, int n2, void *stream);
+
+/* Weights of the transposed convolution used for the input gradient:
+ * wt[tap][ci][co] = w[26 - tap][co][ci] (cin = 32). */
+int pvd_cnn_flip_weights(const void *w, void *wt, void *stream);
+
+/* dz = bf16(dh * (act > 0)) over n voxels x 32 channels; dbias_part[b][c]
+ * receives per-block channel sums of dz (nblocks = ceil(n / 1024)). */
+int pvd_cnn_relu_grad(const float *dh, const void *act, void *dz, float *dbias_part, long long n_voxels,
+                      void *stream);
+
+/* Weight gradient dw[tap][co][ci] = sum_v x[v + d(tap)][ci] dz[v][co] (FP32),
+ * accumulated over voxels in a fixed order (deterministic).  work: FP32 scratch
+ * of pvd_cnn_wgrad_work_floats() floats. */
+size_t pvd_cnn_wgrad_work_floats(int n0, int n1, int n2);
+int pvd_cnn_wgrad(const void *x, int cin, const void *dz, float *dw, float *work, int n0, int n1, int n2,
+                  void *stream);
+
+/* Loss of the multiplicative correction D = Dp (1 + r) against the target Dt on
+ * voxels with weight m: loss = sum m (Dp r - (Dt - Dp))^2 / sum m.
+ * Writes dr (bf16, channel 0 of a 32-channel tensor, others 0) and loss_part
+ * (PVD_CNN_LOSS_BLOCKS rows of 3 floats: partial sums of m*e^2, of m and of dr). */
+#define PVD_CNN_LOSS_BLOCKS 256
+int pvd_cnn_loss(const float *r, const float *dp, const float *dt, const float *m, long long n_voxels,
+                 float inv_count, void *dr, float *loss_part, void *stream);
+
+/* Adam on FP32 master weights; also refreshes the BF16 copy. */
+int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *param_bf16, long long n, float lr,
+                 float beta1, float beta2, float eps, int step, void *stream);
+
+/* Network input: x[v] = bf16(activity*sa, density, dose_plain*sd, 0 x 13). */
+int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, float sa, float sd,
+                        void *x, long long n_voxels, void *stream);
+
+/* Corrected dose with fallback: out = dp*(1+r) where |r| <= bound, else dp;
+ * exceed[0] += number of voxels with |r| > bound and dp >= region_min (FP32 atomics
+ * of whole counts, exact below 2^24 per call). */
+int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, float region_min, float *out,
+                             float *exceed, long long n_voxels, void *stream);
+
+/* Sum rows of a [rows][cols] FP32 array in fixed order into out[cols] (+= if accumulate). */
+int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PVD_LEARNED_H */

@@ -73,7 +73,35 @@ SIGNATURES = {
     "pvd_batch_destroy": (_I, [_P]),
     "pvd_plan_timings": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "pvd_pass_name": (ctypes.c_char_p, [_P, _I]),
+    # include/pvd_learned.h
+    "pvd_mc_dose": (_I, [ctypes.c_void_p, _P, _P, _P, _P, _P]),
+    "pvd_cnn_conv_fwd": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P]),
+    "pvd_cnn_flip_weights": (_I, [_P, _P, _P]),
+    "pvd_cnn_relu_grad": (_I, [_P, _P, _P, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_wgrad_work_floats": (ctypes.c_size_t, [_I, _I, _I]),
+    "pvd_cnn_wgrad": (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _P]),
+    "pvd_cnn_loss": (_I, [_P, _P, _P, _P, ctypes.c_longlong, ctypes.c_float, _P, _P, _P]),
+    "pvd_cnn_adam": (_I, [_P, _P, _P, _P, _P, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, _I, _P]),
+    "pvd_reduce_rows": (_I, [_P, _I, _I, _P, _I, _P]),
+    "pvd_cnn_pack_inputs": (_I, [_P, _P, _P, ctypes.c_float, ctypes.c_float, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, ctypes.c_float, _P, _P, ctypes.c_longlong, _P]),
 }
+
+
+class McParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("voxel_mm", ctypes.c_double), ("histories", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong), ("e_of_u", ctypes.c_void_p), ("n_u", ctypes.c_int),
+                ("e_of_r", ctypes.c_void_p), ("n_r", ctypes.c_int), ("r_max_mm", ctypes.c_double),
+                ("katz_a", ctypes.c_double), ("katz_b", ctypes.c_double), ("katz_c", ctypes.c_double),
+                ("step_mm", ctypes.c_double), ("scatter", ctypes.c_double), ("rho_min", ctypes.c_double),
+                ("active", ctypes.c_void_p), ("n_active", ctypes.c_longlong)]
+
+
+PVD_CNN_EPI_RELU = 0
+PVD_CNN_EPI_RESID_RELU = 1
+PVD_CNN_EPI_LINEAR_F32 = 2
+PVD_CNN_EPI_ACCUM_F32 = 3
 
 _lib = None
 _lock = threading.Lock()

@@ -54,6 +54,22 @@ int hip_fail(hipError_t e, const char *what) {
                 (int)e);
 }
 
+}  // namespace
+
+int pvd_set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int pvd_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
+
+namespace {
+
 #define PVD_HIP(call, what)                         \
     do {                                            \
         hipError_t e_ = (call);                     \

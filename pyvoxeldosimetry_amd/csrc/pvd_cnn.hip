@@ -1,0 +1,541 @@
+// 3-D residual CNN kernels on the gfx950 matrix cores (BF16 in, FP32 accumulate).
+//
+// Layout: activations channels-last BF16 [n0][n1][n2][C] (n2 % 32 == 0), weights
+// BF16 [27][32 cout][cin] with tap = (dx+1)*9 + (dy+1)*3 + (dz+1).
+//
+// conv_fwd  implicit GEMM per output row of 32 voxels along z:
+//           D[32 voxels][32 cout] += A[32 voxels][16 cin] * B[16 cin][32 cout]
+//           with v_mfma_f32_32x32x16_bf16, K-loop over 27 taps x cin/16.
+//           Persistent workgroups (8 waves) keep all weights in LDS and stage a
+//           halo of 2x4 output rows; voxel/cout strides are padded by 16 bytes so
+//           the 16-byte fragment reads of 8 consecutive lanes hit distinct banks.
+// wgrad     D[32 cout][32 cin] per tap += A[32 cout][16 voxels] * B[16 voxels][32 cin]
+//           from channel-major (transposed) LDS copies; the input halo is stored
+//           three times, pre-shifted by dz = -1, 0, +1, so every fragment read is a
+//           16-byte aligned load.  Each wave owns a fixed subset of taps; per-
+//           workgroup partial sums are reduced in a fixed order (deterministic).
+#include "pvd_internal.h"
+
+#include "../../include/pvd_learned.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+// row of accumulator i held by `lane` in the 32x32 MFMA output; its column is lane % 32
+__device__ __forceinline__ int mfma_row(int i, int lane) { return ((i >> 2) << 3) + ((lane >> 5) << 2) + (i & 3); }
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 lds_frag(const char *p) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+struct ConvArgs {
+    const uint16_t *x;
+    const uint16_t *w;
+    const float *bias;
+    const uint16_t *resid;
+    uint16_t *y;
+    uint16_t *act;
+    float *y32;
+    int n0, n1, n2;
+};
+
+constexpr int FWD_BX = 2, FWD_BY = 4, FWD_THREADS = 512;
+constexpr int HX = FWD_BX + 2, HY = FWD_BY + 2, HZ = 34;
+
+template <int CIN>
+constexpr size_t fwd_lds_bytes() {
+    return (size_t)27 * 32 * (CIN * 2 + 16) + (size_t)HX * HY * HZ * (CIN * 2 + 16);
+}
+
+template <int CIN, int EPI>
+__global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
+    extern __shared__ uint4 lds_u4[];
+    char *lds = reinterpret_cast<char *>(lds_u4);
+    constexpr int VS = CIN * 2 + 16;   // bytes per halo voxel
+    constexpr int WS = CIN * 2 + 16;   // bytes per (tap, cout) weight row
+    constexpr int C8 = CIN / 8;        // 16-byte chunks per voxel
+    char *wl = lds;
+    char *hl = lds + 27 * 32 * WS;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 27 * 32 * C8; i += FWD_THREADS) {
+        const int row = i / C8, c8 = i - row * C8;
+        *reinterpret_cast<uint4 *>(wl + row * WS + c8 * 16) = reinterpret_cast<const uint4 *>(a.w)[i];
+    }
+    const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
+    const int total = nbx * nby * nbz;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wx = wave / FWD_BY, wy = wave % FWD_BY;
+    const int m = lane & 31, kh = lane >> 5;
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        __syncthreads();   // weights visible / previous block finished with the halo
+        for (int i = tid; i < HX * HY * HZ * C8; i += FWD_THREADS) {
+            const int c8 = i % C8, vox = i / C8;
+            const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
+                v = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
+            *reinterpret_cast<uint4 *>(hl + vox * VS + c8 * 16) = v;
+        }
+        __syncthreads();
+        const int ox = x0 + wx, oy = y0 + wy;
+        if (ox >= a.n0 || oy >= a.n1) continue;      // wave-uniform
+        f32x16 acc = {};
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+            const char *hrow = hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * VS + kh * 16;
+            const char *wrow = wl + (tap * 32 + m) * WS + kh * 16;
+#pragma unroll
+            for (int cc = 0; cc < CIN / 16; ++cc)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32), acc,
+                                                              0, 0, 0);
+        }
+        const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
+        const int co = m;
+        const float bco = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const long long v = vrow + mfma_row(i, lane);
+            const float zv = acc[i] + bco;
+            if (EPI == PVD_CNN_EPI_RELU) {
+                const uint16_t h = f2bf(fmaxf(zv, 0.0f));
+                a.y[v * 32 + co] = h;
+                if (a.act) a.act[v * 32 + co] = h;
+            } else if (EPI == PVD_CNN_EPI_RESID_RELU) {
+                const float av = fmaxf(zv, 0.0f);
+                a.act[v * 32 + co] = f2bf(av);
+                a.y[v * 32 + co] = f2bf(bf2f(a.resid[v * 32 + co]) + av);
+            } else if (EPI == PVD_CNN_EPI_LINEAR_F32) {
+                if (co == 0) a.y32[v] = zv;
+            } else {
+                a.y32[v * 32 + co] += acc[i];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+struct WgradArgs {
+    const uint16_t *x;
+    const uint16_t *dz;
+    float *work;
+    int n0, n1, n2;
+};
+
+constexpr int WG_BX = 2, WG_BY = 2, WG_THREADS = 512;
+constexpr int WHX = WG_BX + 2, WHY = WG_BY + 2, WROWS = WHX * WHY;   // 16 halo rows
+constexpr int XCS = WROWS * 32 * 2 + 16;     // bytes per cin row of one shifted copy (1040)
+constexpr int XCOPY = 32 * XCS;              // one dz-shifted copy
+constexpr int DCS = WG_BX * WG_BY * 32 * 2 + 16;   // bytes per cout row of dz (272)
+constexpr size_t WG_LDS = 3 * (size_t)XCOPY + 32 * (size_t)DCS;
+constexpr int WG_MAX_BLOCKS = 256;
+
+__device__ __forceinline__ void st_bf16(char *p, uint16_t v) { *reinterpret_cast<uint16_t *>(p) = v; }
+
+template <int CIN>
+__global__ __launch_bounds__(WG_THREADS) void cnn_wgrad(WgradArgs a) {
+    extern __shared__ uint4 lds_u4[];
+    char *lds = reinterpret_cast<char *>(lds_u4);
+    char *xs = lds;                  // [3 dz][32 ci][16 rows][32 z]
+    char *ds = lds + 3 * XCOPY;      // [32 co][4 rows][32 z]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 31, kh = lane >> 5;
+    const int nbx = (a.n0 + WG_BX - 1) / WG_BX, nby = (a.n1 + WG_BY - 1) / WG_BY, nbz = a.n2 >> 5;
+    const int total = nbx * nby * nbz;
+    f32x16 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = f32x16{};
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * WG_BX, y0 = by * WG_BY, z0 = bz * 32;
+        __syncthreads();
+        // input halo rows, z in [z0-1, z0+33), into the three shifted channel-major copies
+        for (int i = tid; i < WROWS * 34 * 4; i += WG_THREADS) {
+            const int c8 = i & 3, vox = i >> 2;           // 4 chunks of 8 channels (cin 32; 16 pads zeros)
+            const int hz = vox % 34, hrow = vox / 34;
+            const int hx = hrow / WHY, hy = hrow % WHY;
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (c8 < CIN / 8 && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
+                v = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
+            const uint16_t h[8] = {(uint16_t)(v.x & 0xffff), (uint16_t)(v.x >> 16), (uint16_t)(v.y & 0xffff),
+                                   (uint16_t)(v.y >> 16),    (uint16_t)(v.z & 0xffff), (uint16_t)(v.z >> 16),
+                                   (uint16_t)(v.w & 0xffff), (uint16_t)(v.w >> 16)};
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {      // copy s holds x at z0 + z + (s - 1)
+                const int z = hz - s;            // gz = z0 + hz - 1 = z0 + z + s - 1
+                if (z < 0 || z >= 32) continue;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    st_bf16(xs + s * XCOPY + (c8 * 8 + j) * XCS + (hrow * 32 + z) * 2, h[j]);
+            }
+        }
+        // output-gradient rows of the block, channel-major
+        for (int i = tid; i < WG_BX * WG_BY * 32 * 4; i += WG_THREADS) {
+            const int c8 = i & 3, vox = i >> 2;
+            const int z = vox & 31, r = vox >> 5;
+            const int ox = x0 + r / WG_BY, oy = y0 + r % WG_BY;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (ox < a.n0 && oy < a.n1)
+                v = reinterpret_cast<const uint4 *>(a.dz + (((long long)ox * a.n1 + oy) * a.n2 + z0 + z) * 32)[c8];
+            const uint16_t h[8] = {(uint16_t)(v.x & 0xffff), (uint16_t)(v.x >> 16), (uint16_t)(v.y & 0xffff),
+                                   (uint16_t)(v.y >> 16),    (uint16_t)(v.z & 0xffff), (uint16_t)(v.z >> 16),
+                                   (uint16_t)(v.w & 0xffff), (uint16_t)(v.w >> 16)};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) st_bf16(ds + (c8 * 8 + j) * DCS + (r * 32 + z) * 2, h[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int tap = wave + 8 * j;
+            if (tap >= 27) break;
+            const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dzs = tap % 3 - 1;
+            const char *xcopy = xs + (dzs + 1) * XCOPY + m * XCS;
+            for (int r = 0; r < WG_BX * WG_BY; ++r) {
+                const int hrow = (r / WG_BY + dx + 1) * WHY + (r % WG_BY + dy + 1);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const bf16x8 A = lds_frag(ds + m * DCS + (r * 32 + c * 16 + kh * 8) * 2);
+                    const bf16x8 B = lds_frag(xcopy + (hrow * 32 + c * 16 + kh * 8) * 2);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // partial dW of this workgroup: work[blockIdx][tap][co][ci], ci < CIN
+    float *out = a.work + (size_t)blockIdx.x * 27 * 32 * CIN;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int tap = wave + 8 * j;
+        if (tap >= 27) break;
+        const int ci = m;
+        if (ci >= CIN) continue;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) out[((size_t)tap * 32 + mfma_row(i, lane)) * CIN + ci] = acc[j][i];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void cnn_flip(const uint16_t *w, uint16_t *wt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;     // over 27*32*32
+    if (i >= 27 * 32 * 32) return;
+    const int ci = i & 31, co = (i >> 5) & 31, tap = i >> 10;
+    // wt[tap][ci][co] = w[26 - tap][co][ci]  (stored with the second index as "cout")
+    wt[((size_t)tap * 32 + ci) * 32 + co] = w[((size_t)(26 - tap) * 32 + co) * 32 + ci];
+}
+
+__global__ __launch_bounds__(256) void cnn_relu_grad(const float *__restrict__ dh, const uint16_t *__restrict__ act,
+                                                     uint16_t *__restrict__ dz, float *__restrict__ part,
+                                                     long long n) {
+    __shared__ float red[8][32];
+    const int c = threadIdx.x & 31, vo = threadIdx.x >> 5;
+    float sum = 0.0f;
+    const long long base = (long long)blockIdx.x * 1024;
+    for (int k = 0; k < 128; ++k) {
+        const long long v = base + k * 8 + vo;
+        if (v < n) {
+            const long long idx = v * 32 + c;
+            const uint16_t av = act[idx];
+            const float g = ((av & 0x8000u) == 0 && av != 0) ? dh[idx] : 0.0f;
+            dz[idx] = f2bf(g);
+            sum += g;
+        }
+    }
+    red[vo][c] = sum;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float s = 0.0f;
+        for (int j = 0; j < 8; ++j) s += red[j][threadIdx.x];
+        part[(size_t)blockIdx.x * 32 + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void cnn_loss(const float *__restrict__ r, const float *__restrict__ dp,
+                                                const float *__restrict__ dt, const float *__restrict__ mk,
+                                                long long n, float inv_count, uint16_t *__restrict__ dr,
+                                                float *__restrict__ part) {
+    __shared__ float red[3][256];
+    float se = 0.0f, sw = 0.0f, sg = 0.0f;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const float w = mk[v];
+        const float e = dp[v] * r[v] - (dt[v] - dp[v]);
+        se += w * e * e;
+        sw += w;
+        const uint16_t g = f2bf(2.0f * w * e * dp[v] * inv_count);
+        sg += bf2f(g);
+        uint4 *row = reinterpret_cast<uint4 *>(dr + v * 32);
+        row[0] = make_uint4((uint32_t)g, 0u, 0u, 0u);
+        row[1] = make_uint4(0u, 0u, 0u, 0u);
+        row[2] = make_uint4(0u, 0u, 0u, 0u);
+        row[3] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    red[0][threadIdx.x] = se;
+    red[1][threadIdx.x] = sw;
+    red[2][threadIdx.x] = sg;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+            red[2][threadIdx.x] += red[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void cnn_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, long long n,
+                                                float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+        p[i] = pi;
+        if (pb) pb[i] = f2bf(pi);
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_rows(const float *__restrict__ part, int rows, int cols,
+                                                   float *__restrict__ out, int accumulate) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += part[(size_t)r * cols + c];
+    out[c] = accumulate ? out[c] + s : s;
+}
+
+__global__ __launch_bounds__(256) void cnn_pack(const float *__restrict__ a, const float *__restrict__ d,
+                                                const float *__restrict__ dp, float sa, float sd,
+                                                uint16_t *__restrict__ x, long long n) {
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const uint32_t c0 = f2bf(a[v] * sa), c1 = f2bf(d[v]), c2 = f2bf(dp[v] * sd);
+        uint4 *row = reinterpret_cast<uint4 *>(x + v * 16);
+        row[0] = make_uint4(c0 | (c1 << 16), c2, 0u, 0u);
+        row[1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+__global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, const float *__restrict__ r,
+                                                 float bound, float region_min, float *__restrict__ out,
+                                                 float *__restrict__ exceed, long long n) {
+    __shared__ float red[256];
+    float cnt = 0.0f;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const float rv = r[v];
+        const bool ok = fabsf(rv) <= bound;      // NaN fails too
+        out[v] = ok ? dp[v] * (1.0f + rv) : dp[v];
+        if (!ok && dp[v] >= region_min) cnt += 1.0f;
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] > 0.0f) atomicAdd(exceed, red[0]);
+}
+
+int g_num_cus = 0;
+
+int num_cus() {
+    if (g_num_cus == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            g_num_cus = v;
+        else
+            g_num_cus = 256;
+    }
+    return g_num_cus;
+}
+
+template <int CIN, int EPI>
+hipError_t launch_fwd(const ConvArgs &a, int blocks, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_fwd<CIN, EPI>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd_lds_bytes<CIN>());
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((cnn_conv_fwd<CIN, EPI>), dim3(blocks), dim3(FWD_THREADS), fwd_lds_bytes<CIN>(), st, a);
+    return hipGetLastError();
+}
+
+template <int CIN>
+hipError_t launch_wgrad(const WgradArgs &a, int blocks, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_wgrad<CIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)WG_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((cnn_wgrad<CIN>), dim3(blocks), dim3(WG_THREADS), WG_LDS, st, a);
+    return hipGetLastError();
+}
+
+int check_grid(int n0, int n1, int n2) {
+    if (n0 < 1 || n1 < 1 || n2 < 32 || (n2 & 31))
+        return pvd_set_error(PVD_EINVAL, "CNN grid %dx%dx%d: need n2 a positive multiple of 32", n0, n1, n2);
+    return PVD_OK;
+}
+
+int wgrad_blocks(int n0, int n1, int n2) {
+    const long long total = (long long)((n0 + WG_BX - 1) / WG_BX) * ((n1 + WG_BY - 1) / WG_BY) * (n2 >> 5);
+    return (int)(total < WG_MAX_BLOCKS ? total : WG_MAX_BLOCKS);
+}
+
+}  // namespace
+
+// =================================================================================
+extern "C" {
+
+int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, int epi, const void *resid, void *y,
+                     void *act, float *y32, int n0, int n1, int n2, void *stream) {
+    if (int rc = check_grid(n0, n1, n2)) return rc;
+    if (!x || !w) return pvd_set_error(PVD_EINVAL, "null input or weights");
+    if (cin != 16 && cin != 32) return pvd_set_error(PVD_EINVAL, "cin must be 16 or 32 (got %d)", cin);
+    switch (epi) {
+        case PVD_CNN_EPI_RELU: if (!y) return pvd_set_error(PVD_EINVAL, "null output"); break;
+        case PVD_CNN_EPI_RESID_RELU:
+            if (!y || !act || !resid) return pvd_set_error(PVD_EINVAL, "residual layer needs y, act and resid");
+            break;
+        case PVD_CNN_EPI_LINEAR_F32:
+        case PVD_CNN_EPI_ACCUM_F32: if (!y32) return pvd_set_error(PVD_EINVAL, "null FP32 output"); break;
+        default: return pvd_set_error(PVD_EINVAL, "unknown epilogue %d", epi);
+    }
+    ConvArgs a{(const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)resid, (uint16_t *)y,
+               (uint16_t *)act, y32, n0, n1, n2};
+    const long long total = (long long)((n0 + FWD_BX - 1) / FWD_BX) * ((n1 + FWD_BY - 1) / FWD_BY) * (n2 >> 5);
+    const int blocks = (int)(total < num_cus() ? total : num_cus());
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    if (cin == 16) {
+        switch (epi) {
+            case 0: e = launch_fwd<16, 0>(a, blocks, st); break;
+            case 1: e = launch_fwd<16, 1>(a, blocks, st); break;
+            case 2: e = launch_fwd<16, 2>(a, blocks, st); break;
+            default: e = launch_fwd<16, 3>(a, blocks, st); break;
+        }
+    } else {
+        switch (epi) {
+            case 0: e = launch_fwd<32, 0>(a, blocks, st); break;
+            case 1: e = launch_fwd<32, 1>(a, blocks, st); break;
+            case 2: e = launch_fwd<32, 2>(a, blocks, st); break;
+            default: e = launch_fwd<32, 3>(a, blocks, st); break;
+        }
+    }
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch CNN convolution");
+}
+
+int pvd_cnn_flip_weights(const void *w, void *wt, void *stream) {
+    if (!w || !wt) return pvd_set_error(PVD_EINVAL, "null argument");
+    hipLaunchKernelGGL(cnn_flip, dim3((27 * 32 * 32 + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       (const uint16_t *)w, (uint16_t *)wt);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch weight flip");
+}
+
+int pvd_cnn_relu_grad(const float *dh, const void *act, void *dz, float *dbias_part, long long n, void *stream) {
+    if (!dh || !act || !dz || !dbias_part || n < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    const long long blocks = (n + 1023) / 1024;
+    if (blocks > 0x7fffffffLL) return pvd_set_error(PVD_ENOTSUP, "too many voxels");
+    hipLaunchKernelGGL(cnn_relu_grad, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dh,
+                       (const uint16_t *)act, (uint16_t *)dz, dbias_part, n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch ReLU gradient");
+}
+
+size_t pvd_cnn_wgrad_work_floats(int n0, int n1, int n2) {
+    if (n0 < 1 || n1 < 1 || n2 < 32) return 0;
+    return (size_t)wgrad_blocks(n0, n1, n2) * 27 * 32 * 32;
+}
+
+int pvd_cnn_wgrad(const void *x, int cin, const void *dz, float *dw, float *work, int n0, int n1, int n2,
+                  void *stream) {
+    if (int rc = check_grid(n0, n1, n2)) return rc;
+    if (!x || !dz || !dw || !work) return pvd_set_error(PVD_EINVAL, "null argument");
+    if (cin != 16 && cin != 32) return pvd_set_error(PVD_EINVAL, "cin must be 16 or 32 (got %d)", cin);
+    WgradArgs a{(const uint16_t *)x, (const uint16_t *)dz, work, n0, n1, n2};
+    const int blocks = wgrad_blocks(n0, n1, n2);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = cin == 16 ? launch_wgrad<16>(a, blocks, st) : launch_wgrad<32>(a, blocks, st);
+    if (e != hipSuccess) return pvd_hip_error(e, "launch weight gradient");
+    const int cols = 27 * 32 * cin;
+    hipLaunchKernelGGL(reduce_rows, dim3((cols + 255) / 256), dim3(256), 0, st, (const float *)work, blocks, cols, dw, 0);
+    e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch reduction");
+}
+
+int pvd_cnn_loss(const float *r, const float *dp, const float *dt, const float *m, long long n, float inv_count,
+                 void *dr, float *loss_part, void *stream) {
+    if (!r || !dp || !dt || !m || !dr || !loss_part || n < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    hipLaunchKernelGGL(cnn_loss, dim3(PVD_CNN_LOSS_BLOCKS), dim3(256), 0, (hipStream_t)stream, r, dp, dt, m, n, inv_count,
+                       (uint16_t *)dr, loss_part);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch loss");
+}
+
+int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *param_bf16, long long n, float lr,
+                 float beta1, float beta2, float eps, int step, void *stream) {
+    if (!param || !grad || !m || !v || n < 1 || step < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    const float c1 = 1.0f / (1.0f - powf(beta1, (float)step));
+    const float c2 = 1.0f / (1.0f - powf(beta2, (float)step));
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(cnn_adam, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, param, grad, m, v,
+                       (uint16_t *)param_bf16, n, lr, beta1, beta2, eps, c1, c2);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch Adam");
+}
+
+int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, float sa, float sd,
+                        void *x, long long n, void *stream) {
+    if (!activity || !density || !dose_plain || !x || n < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(cnn_pack, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, activity, density,
+                       dose_plain, sa, sd, (uint16_t *)x, n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch input packing");
+}
+
+int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, float region_min, float *out,
+                             float *exceed, long long n, void *stream) {
+    if (!dp || !r || !out || !exceed || n < 1 || !(bound >= 0.0f)) return pvd_set_error(PVD_EINVAL, "bad argument");
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(cnn_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, r, bound, region_min,
+                       out, exceed, n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch correction");
+}
+
+int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream) {
+    if (!part || !out || rows < 1 || cols < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    hipLaunchKernelGGL(reduce_rows, dim3((cols + 255) / 256), dim3(256), 0, (hipStream_t)stream, part, rows, cols, out,
+                       accumulate);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch reduction");
+}
+
+}  // extern "C"

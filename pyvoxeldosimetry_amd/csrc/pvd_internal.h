@@ -284,5 +284,10 @@ __device__ __forceinline__ void fft_lines(float2 *__restrict__ s, int lnl, int l
     }
 }
 
+// Record an error message for pvd_last_error() and return `code` (pvd_api.hip).
+int pvd_set_error(int code, const char *fmt, ...);
+// Map a HIP error to PVD_ENOMEM / PVD_EHIP with a message.
+int pvd_hip_error(hipError_t e, const char *what);
+
 // rho_w / rho with rho_w = 1 g/cc; 0 where rho <= 0 (declared epilogue).
 __device__ __forceinline__ float density_factor(float rho) { return rho > 0.0f ? 1.0f / rho : 0.0f; }

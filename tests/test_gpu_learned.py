@@ -1,0 +1,288 @@
+"""GPU tests of the learned correction: MFMA CNN kernels vs the FP64 oracle, and
+invariants of the Monte Carlo.
+
+Tolerances: the GPU multiplies BF16 values exactly and accumulates in FP32, so
+layer outputs computed from the same BF16 operands agree with FP64 to ~1e-5
+relative (FP32 outputs) or to BF16 rounding (~4e-3, BF16 outputs).  Whole-
+network gradients also see BF16 rounding of every stored activation, so they
+are compared by relative Frobenius norm (<= 3e-2).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cnn as ocnn
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-case]", *a, file=sys.stderr, flush=True)
+
+
+def bf(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(ocnn.bf16_round(a))).to(torch.bfloat16).cuda()
+
+
+def rel_fro(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+SHAPE = (3, 5, 64)
+
+
+def _conv(lib, x, cin, w, bias, epi, shape, y=None, act=None, resid=None, y32=None):
+    import ctypes
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)  # noqa: E731
+    nat.check(lib.pvd_cnn_conv_fwd(p(x), cin, p(w), p(bias), epi, p(resid), p(y), p(act), p(y32), *shape,
+                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "conv")
+
+
+@pytest.mark.parametrize("cin", [16, 32])
+def test_conv_forward_all_epilogues(cin):
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    _say("cnn conv fwd", cin)
+    lib = nat.load()
+    rng = np.random.default_rng(cin)
+    n = int(np.prod(SHAPE))
+    x = ocnn.bf16_round(rng.normal(size=SHAPE + (cin,)).astype(np.float32))
+    w = ocnn.bf16_round(rng.normal(scale=0.1, size=(27, 32, cin)).astype(np.float32))
+    b = rng.normal(scale=0.1, size=32).astype(np.float32)
+    z = ocnn.conv3(x, w) + b
+    xt, wt, bt = bf(x).reshape(n, cin), bf(w), torch.from_numpy(b).cuda()
+    # relu
+    y = torch.empty((n, 32), dtype=torch.bfloat16, device="cuda")
+    act = torch.empty_like(y)
+    _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RELU, SHAPE, y=y, act=act)
+    got = y.float().cpu().numpy().reshape(SHAPE + (32,))
+    np.testing.assert_allclose(got, np.maximum(z, 0), rtol=8e-3, atol=1e-3)
+    assert torch.equal(y, act)
+    # linear FP32 (channel 0)
+    r = torch.empty(n, dtype=torch.float32, device="cuda")
+    _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_LINEAR_F32, SHAPE, y32=r)
+    np.testing.assert_allclose(r.cpu().numpy().reshape(SHAPE), z[..., 0], rtol=1e-5, atol=1e-4)
+    # accumulate FP32 (no bias)
+    acc = torch.ones((n, 32), dtype=torch.float32, device="cuda")
+    _conv(lib, xt, cin, wt, None, nat.PVD_CNN_EPI_ACCUM_F32, SHAPE, y32=acc)
+    np.testing.assert_allclose(acc.cpu().numpy().reshape(SHAPE + (32,)), 1.0 + ocnn.conv3(x, w), rtol=1e-5, atol=1e-4)
+    if cin == 32:
+        resid = bf(rng.normal(size=SHAPE + (32,)).astype(np.float32)).reshape(n, 32)
+        y2 = torch.empty_like(y)
+        _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, SHAPE, y=y2, act=act, resid=resid)
+        exp = resid.float().cpu().numpy().reshape(SHAPE + (32,)) + np.maximum(z, 0)
+        np.testing.assert_allclose(y2.float().cpu().numpy().reshape(SHAPE + (32,)), exp, rtol=8e-3, atol=2e-3)
+        np.testing.assert_allclose(act.float().cpu().numpy().reshape(SHAPE + (32,)), np.maximum(z, 0), rtol=8e-3,
+                                   atol=1e-3)
+
+
+@pytest.mark.parametrize("cin", [16, 32])
+def test_weight_and_input_gradients(cin):
+    import ctypes
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    _say("cnn wgrad/dgrad", cin)
+    lib = nat.load()
+    rng = np.random.default_rng(100 + cin)
+    shape = (5, 3, 64)
+    n = int(np.prod(shape))
+    x = ocnn.bf16_round(rng.normal(size=shape + (cin,)).astype(np.float32))
+    dz = ocnn.bf16_round(rng.normal(size=shape + (32,)).astype(np.float32))
+    work = torch.empty(int(lib.pvd_cnn_wgrad_work_floats(*shape)), dtype=torch.float32, device="cuda")
+    dw = torch.empty(27 * 32 * cin, dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    xt, dzt = bf(x), bf(dz)          # keep the device tensors alive across the asynchronous calls
+    nat.check(lib.pvd_cnn_wgrad(ctypes.c_void_p(xt.data_ptr()), cin, ctypes.c_void_p(dzt.data_ptr()),
+                                ctypes.c_void_p(dw.data_ptr()), ctypes.c_void_p(work.data_ptr()), *shape, st), "wgrad")
+    exp = ocnn.conv3_weight_grad(x, dz)
+    got = dw.cpu().numpy().reshape(27, 32, cin)
+    np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-3 * np.abs(exp).max())
+    # the same call twice gives identical bits (fixed-order reduction)
+    dw2 = torch.empty_like(dw)
+    nat.check(lib.pvd_cnn_wgrad(ctypes.c_void_p(xt.data_ptr()), cin, ctypes.c_void_p(dzt.data_ptr()),
+                                ctypes.c_void_p(dw2.data_ptr()), ctypes.c_void_p(work.data_ptr()), *shape, st), "wgrad")
+    assert torch.equal(dw, dw2)
+    if cin == 32:
+        w = ocnn.bf16_round(rng.normal(scale=0.1, size=(27, 32, 32)).astype(np.float32))
+        wt = torch.empty(27 * 32 * 32, dtype=torch.bfloat16, device="cuda")
+        wb = bf(w)
+        nat.check(lib.pvd_cnn_flip_weights(ctypes.c_void_p(wb.data_ptr()), ctypes.c_void_p(wt.data_ptr()), st),
+                  "flip")
+        dx = torch.zeros((n, 32), dtype=torch.float32, device="cuda")
+        _conv(lib, dzt.reshape(n, 32), 32, wt, None, nat.PVD_CNN_EPI_ACCUM_F32, shape, y32=dx)
+        np.testing.assert_allclose(dx.cpu().numpy().reshape(shape + (32,)), ocnn.conv3_input_grad(dz, w), rtol=1e-4,
+                                   atol=1e-4)
+
+
+def test_network_gradients_match_oracle():
+    import torch
+    from pyvoxeldosimetry_amd.learned.cnn import ResidualCNN
+    _say("cnn full gradients")
+    shape = (4, 4, 32)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(3)
+    net = ResidualCNN(seed=5)
+    params = net.params_numpy()
+    x = np.zeros(shape + (16,), np.float32)
+    x[..., :3] = rng.uniform(0, 1, size=shape + (3,))
+    x = ocnn.bf16_round(x)
+    dp = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+    dt = (dp * rng.uniform(0.8, 1.2, size=shape)).astype(np.float32)
+    m = (rng.random(shape) > 0.3).astype(np.float32)
+    xt = bf(x).reshape(n, 16)
+    r = net.forward(xt, shape)
+    r_exp, cache = ocnn.forward(params, x.astype(np.float64))
+    assert rel_fro(r.cpu().numpy().reshape(shape), r_exp) < 2e-2
+    sums = net.loss_and_grad(*(torch.from_numpy(a.reshape(-1)).cuda() for a in (dp, dt, m)), shape)
+    loss = (sums[0] / sums[1]).item()
+    loss_exp, dr = ocnn.loss_and_grad_r(r_exp, dp, dt, m)
+    assert loss == pytest.approx(loss_exp, rel=3e-2)
+    net.backward(xt, shape)
+    grads = ocnn.backward(params, cache, dr)
+    for l in range(6):
+        gw = net.grad_w(l).cpu().numpy().reshape(grads[l][0].shape)
+        gb = net.grad_b(l).cpu().numpy()
+        assert rel_fro(gw, grads[l][0]) < 3e-2, (l, rel_fro(gw, grads[l][0]))
+        assert rel_fro(gb, grads[l][1]) < 3e-2, (l, rel_fro(gb, grads[l][1]))
+
+
+def test_adam_matches_reference_update():
+    import torch
+    from pyvoxeldosimetry_amd.learned.cnn import ResidualCNN
+    _say("adam")
+    net = ResidualCNN(seed=1)
+    p0 = net.master.cpu().numpy().astype(np.float64)
+    g = np.random.default_rng(4).normal(size=p0.shape).astype(np.float32)
+    net.grad.copy_(torch.from_numpy(g))
+    net.adam_step(lr=1e-2)
+    net.grad.copy_(torch.from_numpy(g * 0.5))
+    net.adam_step(lr=1e-2)
+    m = v = 0.0
+    p = p0.copy()
+    for t, gg in ((1, g.astype(np.float64)), (2, 0.5 * g.astype(np.float64))):
+        m = 0.9 * m + 0.1 * gg
+        v = 0.999 * v + 0.001 * gg * gg
+        p = p - 1e-2 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    np.testing.assert_allclose(net.master.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(net.bf16.float().cpu().numpy(), ocnn.bf16_round(net.master.cpu().numpy()))
+
+
+# ----------------------------------------------------------------- Monte Carlo
+def test_mc_energy_conservation_and_escape():
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import mean_energy
+    _say("mc conservation")
+    mc = MonteCarlo("Y90")
+    rng = np.random.default_rng(0)
+    a = torch.from_numpy(rng.uniform(0, 1, size=(20, 18, 16)).astype(np.float32)).cuda()
+    rho = torch.from_numpy(rng.choice([0.3, 1.0, 1.85], size=(20, 18, 16)).astype(np.float32)).cuda()
+    e = mc.edep(a, rho, 2.0, histories=64, seed=3)
+    emitted, deposited, escaped = mc.last_totals.cpu().numpy()
+    assert abs(deposited + escaped - emitted) / emitted < 1e-5
+    assert abs(e.double().sum().item() - deposited) / deposited < 1e-4
+    assert 0.0 < escaped < emitted
+    # emitted energy per decay matches the spectrum mean (statistically)
+    assert emitted / a.double().sum().item() == pytest.approx(mean_energy("Y90"), rel=1e-2)
+
+
+def test_mc_density_voxel_scaling_invariance():
+    """(rho, voxel) -> (rho/2, 2 voxel) gives the same deposit map (same random numbers)."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    _say("mc scaling invariance")
+    mc = MonteCarlo("Y90")
+    rng = np.random.default_rng(1)
+    a = torch.from_numpy(rng.uniform(0, 1, size=(16, 16, 16)).astype(np.float32)).cuda()
+    rho = torch.from_numpy(rng.uniform(0.5, 2.0, size=(16, 16, 16)).astype(np.float32)).cuda()
+    e1 = mc.edep(a, rho, 1.0, histories=16, seed=9).double()
+    e2 = mc.edep(a, rho * 0.5, 2.0, histories=16, seed=9).double()
+    assert ((e1 - e2).abs().max() / e1.abs().max()).item() < 1e-4
+
+
+def test_mc_point_kernel_is_isotropic_and_consistent():
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import mean_energy
+    _say("mc point kernel")
+    mc = MonteCarlo("Y90")
+    k = mc.point_kernel(2.0, 15, histories=1 << 20)
+    assert k.sum() == pytest.approx(mean_energy("Y90"), rel=2e-2)      # nearly nothing escapes 15 voxels
+    c = 7
+    assert k[c, c, c] == k.max()
+    # monotone fall-off along an axis
+    prof = k[c, c, c:]
+    assert np.all(np.diff(prof[:6]) < 0)
+
+
+# ----------------------------------------------------------------- learned pipeline
+def test_correction_fallback_rule():
+    """Out-of-bound corrections injected from the host fall back to the plain dose."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, synth_phantom
+    _say("fallback")
+    corr = LearnedCorrector("Y90", voxel_mm=2.0, kernel_size=11, kernel_histories=1 << 20)
+    act, rho = synth_phantom(5, (32, 32, 32), corr.device)
+    dp = corr.plain(act, rho).clone()
+    r = torch.full(act.shape, 0.1, device="cuda")
+    c = corr.correct(act, rho, r_override=r)
+    assert not c.fallback and c.exceed == 0
+    torch.testing.assert_close(c.dose, dp * 1.1, rtol=1e-6, atol=0)
+    r_bad = r.clone()
+    r_bad[0, 0, 0] = 7.0                                   # one voxel out of bound: voxelwise fallback
+    c = corr.correct(act, rho, r_override=r_bad)
+    assert c.dose[0, 0, 0].item() == pytest.approx(dp[0, 0, 0].item())
+    r_bad = torch.full(act.shape, -3.0, device="cuda")     # everything out of bound: whole-volume fallback
+    c = corr.correct(act, rho, r_override=r_bad)
+    assert c.fallback and c.exceed > 0
+    torch.testing.assert_close(c.dose, dp)
+
+
+def test_training_reduces_error_against_monte_carlo():
+    """Short training at 64^3: the loss falls and the corrected dose is closer to
+    held-out Monte Carlo than the plain convolution."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, evaluate
+    from pyvoxeldosimetry_amd.learned.train import build_training_set, train
+    _say("short training 64^3")
+    torch.manual_seed(0)
+    shape = (64, 64, 64)
+    corr = LearnedCorrector("Y90", voxel_mm=2.0, kernel_size=13, kernel_histories=1 << 23, seed=1)
+    before = evaluate(corr, range(20000, 20004), shape, histories=1024)
+    assert before["cnn_mean"] == pytest.approx(before["plain_mean"], rel=1e-6)   # untrained: r = 0
+    data = build_training_set(corr, 96, shape, histories=1024)
+    hist, _ = train(corr, data, epochs=8, lr=2e-3)
+    after = evaluate(corr, range(20000, 20004), shape, histories=1024)
+    _say(f"loss per epoch {['%.3e' % h for h in hist]}")
+    _say(f"plain mean err {after['plain_mean']:.4f} -> cnn {after['cnn_mean']:.4f}; "
+         f"within 2%: plain {after['plain_within']:.3f} -> cnn {after['cnn_within']:.3f}")
+    assert hist[-1] < hist[0]
+    assert after["cnn_mean"] < after["plain_mean"]
+    assert after["cnn_within"] > after["plain_within"]
+
+
+def test_plain_dose_matches_monte_carlo_in_water():
+    """Homogeneous water: the convolution with the Monte Carlo point kernel reproduces
+    the Monte Carlo (validates the kernel and the density epilogue at rho = 1)."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, synth_phantom, voxel_errors
+    _say("plain vs MC in water")
+    corr = LearnedCorrector("Y90", voxel_mm=2.0, kernel_size=13, kernel_histories=1 << 23)
+    act, _ = synth_phantom(7, (48, 48, 64), corr.device)
+    rho = torch.ones_like(act)
+    ref = corr.mc_dose(act, rho, histories=2048, seed=3)
+    ref2 = corr.mc_dose(act, rho, histories=2048, seed=4)
+    plain = corr.plain(act, rho)
+    region = ref >= 0.1 * ref.max()
+    rel = voxel_errors(plain, ref)
+    floor = voxel_errors(ref2, ref)                # two independent MC runs: sqrt(2) x the noise
+    bias = ((plain[region] - ref[region]) / ref[region]).mean().item()
+    _say(f"water: plain-vs-MC mean |err| {rel.mean().item():.4f}, MC-vs-MC {floor.mean().item():.4f}, "
+         f"signed bias {bias:+.4f}")
+    assert abs(bias) < 0.005
+    assert rel.mean().item() < 0.85 * floor.mean().item()     # noise-limited: ~1/sqrt(2) of the MC-MC spread
