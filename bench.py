@@ -32,17 +32,20 @@ def pass_bytes(plan, T: int, density: bool):
     FZ reads T FP32 maps, writes the half spectrum Nx*Ny*H complex64; FY reads
     that and writes Nx*Py*H; X reads Nx*Py*H plus the kernel spectrum
     Px*Py*H and writes Nx*Py*H; IY reads Nx*Py*H, writes Nx*Ny*H; IZ reads
-    Nx*Ny*H (+ FP32 density) and writes the FP32 dose.
+    Nx*Ny*H (+ FP32 density) and writes the FP32 dose.  A mirror-symmetric
+    kernel has a real, even spectrum: X then reads only its FP32 octant
+    (Px/2+1)*(Py/2+1)*H.
     """
     nx, ny, nz = plan.shape
     px, py, pz = plan.fft_n
     h = pz // 2 + 1
     n = nx * ny * nz
     c = 8
+    spec = 4 * (px // 2 + 1) * (py // 2 + 1) * h if plan.symmetric_spectrum else c * px * py * h
     return {
         "fft_z_forward": 4 * T * n + c * nx * ny * h,
         "fft_y_forward": c * nx * ny * h + c * nx * py * h,
-        "fft_x_spectrum_multiply": c * nx * py * h * 2 + c * px * py * h,
+        "fft_x_spectrum_multiply": c * nx * py * h * 2 + spec,
         "fft_y_inverse": c * nx * py * h + c * nx * ny * h,
         "fft_z_inverse_epilogue": c * nx * ny * h + 4 * n * (2 if density else 1),
     }
@@ -170,7 +173,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded CT phantom, seed 2)",
         "config": {"workload": "ct single volume 256x256x200 f32 @2.0 mm, Lu-177 31^3 kernel, density map, "
                                "reference (circular) convolution", "algo": plan.algo,
-                   "fft_n": list(plan.fft_n), "mode": "reference"},
+                   "fft_n": list(plan.fft_n), "mode": "reference", "symmetric_spectrum": plan.symmetric_spectrum},
         "roofline": roof, "operator_roofline": operator_roof,
         "per_pass_ms": {k: round(v, 5) for k, v in acc.items()},
     }

@@ -42,6 +42,7 @@ extern "C" {
 #define PVD_ALGO_DIRECT 2   /* LDS-tiled direct convolution (kernel extent <= 16) */
 
 #define PVD_FLAG_PROFILE 1  /* record HIP events around every pass (pvd_plan_timings) */
+#define PVD_FLAG_GENERAL_SPECTRUM 2  /* keep the full complex spectrum even for symmetric kernels */
 
 #define PVD_MAX_TIMEPOINTS 16
 #define PVD_MAX_FFT_LEN    2048
@@ -60,6 +61,7 @@ typedef struct pvd_plan_params {
 
 typedef struct pvd_plan_info {
     int algo;               /* resolved PVD_ALGO_FFT or PVD_ALGO_DIRECT          */
+    int symmetric_spectrum; /* 1: real-octant kernel spectrum (after set_kernel) */
     int fft_n[3];           /* FFT lengths actually used                         */
     int kernel_used[3];     /* kernel extent after the reference's crop          */
     int n_passes;           /* kernel launches per dose call                     */

@@ -24,6 +24,7 @@ PVD_ALGO_AUTO = 0
 PVD_ALGO_FFT = 1
 PVD_ALGO_DIRECT = 2
 PVD_FLAG_PROFILE = 1
+PVD_FLAG_GENERAL_SPECTRUM = 2
 PVD_MAX_TIMEPOINTS = 16
 PVD_MAX_FFT_LEN = 2048
 
@@ -41,8 +42,8 @@ class PlanParams(ctypes.Structure):
 
 
 class PlanInfo(ctypes.Structure):
-    _fields_ = [("algo", ctypes.c_int), ("fft_n", ctypes.c_int * 3), ("kernel_used", ctypes.c_int * 3),
-                ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+    _fields_ = [("algo", ctypes.c_int), ("symmetric_spectrum", ctypes.c_int), ("fft_n", ctypes.c_int * 3),
+                ("kernel_used", ctypes.c_int * 3), ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
 
 
 class DpkTerm(ctypes.Structure):

@@ -25,7 +25,8 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes);
 hipError_t pvd_launch_dpk(double v, const int g[3], const pvd_dpk_term *terms, int nterms, double factor,
                           double *out, hipStream_t st);
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
-                               int Hp, const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st);
+                               int Hp, int sym, const double *kernel, double2 *T1, double2 *T2, float2 *S,
+                               hipStream_t st);
 size_t pvd_direct_lds_bytes(const int k[3]);
 hipError_t pvd_direct_set_lds_limit(size_t bytes);
 hipError_t pvd_launch_wsum(const float *const *maps, const float *w, int T, size_t n, float *out, hipStream_t st);
@@ -95,7 +96,11 @@ int factor_radices(int n, unsigned char *radix) {
     int ns = 0;
     int p2 = 0;
     while (n % 2 == 0) { n /= 2; ++p2; }
-    while (p2 >= 4) { radix[ns++] = 16; p2 -= 4; }
+#ifndef PVD_MAX_RADIX2
+#define PVD_MAX_RADIX2 16   // largest power-of-two radix used by the Stockham plans
+#endif
+    while (PVD_MAX_RADIX2 >= 16 && p2 >= 4) { radix[ns++] = 16; p2 -= 4; }
+    while (p2 >= 3) { radix[ns++] = 8; p2 -= 3; }
     if (p2 == 3) radix[ns++] = 8;
     else if (p2 == 2) radix[ns++] = 4;
     else if (p2 == 1) radix[ns++] = 2;
@@ -266,11 +271,11 @@ int fft_setup(pvd_plan *p) {
     v.H = Pz / 2 + 1;
     v.Hp = round_up(v.H, 16);
     int lz, bz; size_t ldsz;
-    pick_tile(Pz, 16, 1, &lz, &bz, &ldsz);
+    pick_tile(Pz, PVD_TILE_Z, 1, &lz, &bz, &ldsz);
     int tky, by; size_t ldsy;
-    pick_tile(Py, 16, 0, &tky, &by, &ldsy);
+    pick_tile(Py, PVD_TILE_Y, 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
-    pick_tile(Px, 16, 0, &tkx, &bx, &ldsx);
+    pick_tile(Px, PVD_TILE_X, 0, &tkx, &bx, &ldsx);
     v.llz = ilog2(lz);
     v.ltky = ilog2(tky);
     v.ltkx = ilog2(tkx);
@@ -458,6 +463,7 @@ int pvd_plan_create(pvd_plan **out, const pvd_plan_params *q) {
 int pvd_plan_info_get(const pvd_plan *p, pvd_plan_info *info) {
     if (!p || !info) return fail(PVD_EINVAL, "null argument");
     info->algo = p->algo;
+    info->symmetric_spectrum = p->vol.sym;
     for (int a = 0; a < 3; ++a) {
         info->fft_n[a] = p->algo == PVD_ALGO_FFT ? p->P[a] : 0;
         info->kernel_used[a] = p->Ku[a];
@@ -501,13 +507,41 @@ int pvd_plan_set_kernel(pvd_plan *p, const double *kernel, void *stream) {
         p->has_kernel = true;
         return PVD_OK;
     }
+    // An odd, uncropped kernel that is mirror-symmetric on every axis, placed with its
+    // centre at the FFT origin, has a real spectrum even in kx and ky: keep only that
+    // octant (1/16 of the complex bytes) and recover the reference's origin-anchored
+    // result by the output roll (off = -K//2 mod P).
+    bool sym = !(p->prm.flags & PVD_FLAG_GENERAL_SPECTRUM);
+    for (int a = 0; a < 3 && sym; ++a) sym = (p->K[a] & 1) && p->Ku[a] == p->K[a] && p->lo[a] == 0;
+    if (sym) {
+        const size_t nk = (size_t)p->K[0] * p->K[1] * p->K[2];
+        std::vector<double> h(nk);
+        PVD_HIP(hipMemcpyAsync(h.data(), kernel, sizeof(double) * nk, hipMemcpyDeviceToHost, st), "download kernel");
+        PVD_HIP(hipStreamSynchronize(st), "synchronise");
+        const int K0 = p->K[0], K1 = p->K[1], K2 = p->K[2];
+        for (int i = 0; i < K0 && sym; ++i)
+            for (int j = 0; j < K1 && sym; ++j)
+                for (int l = 0; l < K2 && sym; ++l) {
+                    const double v = h[((size_t)i * K1 + j) * K2 + l];
+                    sym = v == h[((size_t)(K0 - 1 - i) * K1 + j) * K2 + l] &&
+                          v == h[((size_t)i * K1 + (K1 - 1 - j)) * K2 + l] &&
+                          v == h[((size_t)i * K1 + j) * K2 + (K2 - 1 - l)];
+                }
+    }
+    const bool ref = p->prm.mode == PVD_MODE_REFERENCE;
+    for (int a = 0; a < 3; ++a) {
+        if (ref) p->s[a] = sym ? p->K[a] / 2 : 0;
+        const int P = p->P[a];
+        p->vol.ax[a].off = ref ? ((P - (p->s[a] % P)) % P) : 0;
+    }
+    p->vol.sym = sym ? 1 : 0;
     const int H = p->P[2] / 2 + 1;
     double2 *T1 = nullptr, *T2 = nullptr;
     const size_t n1 = (size_t)p->Ku[0] * p->Ku[1] * H, n2 = (size_t)p->Ku[0] * p->P[1] * H;
     PVD_HIP(hipMalloc(&T1, sizeof(double2) * n1), "allocate spectrum scratch");
     hipError_t e = hipMalloc(&T2, sizeof(double2) * n2);
     if (e != hipSuccess) { (void)hipFree(T1); return hip_fail(e, "allocate spectrum scratch"); }
-    e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, p->vol.Hp, kernel, T1, T2, p->spec, st);
+    e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, p->vol.Hp, sym ? 1 : 0, kernel, T1, T2, p->spec, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(T1);
     (void)hipFree(T2);

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
                 if (a.act) a.act[v * 32 + co] = h;
             } else if (EPI == PVD_CNN_EPI_RESID_RELU) {
                 const float av = fmaxf(zv, 0.0f);
-                a.act[v * 32 + co] = f2bf(av);
+                if (a.act) a.act[v * 32 + co] = f2bf(av);
                 a.y[v * 32 + co] = f2bf(bf2f(a.resid[v * 32 + co]) + av);
             } else if (EPI == PVD_CNN_EPI_LINEAR_F32) {
                 if (co == 0) a.y32[v] = zv;
@@ -417,7 +417,7 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
     switch (epi) {
         case PVD_CNN_EPI_RELU: if (!y) return pvd_set_error(PVD_EINVAL, "null output"); break;
         case PVD_CNN_EPI_RESID_RELU:
-            if (!y || !act || !resid) return pvd_set_error(PVD_EINVAL, "residual layer needs y, act and resid");
+            if (!y || !resid) return pvd_set_error(PVD_EINVAL, "residual layer needs y and resid");
             break;
         case PVD_CNN_EPI_LINEAR_F32:
         case PVD_CNN_EPI_ACCUM_F32: if (!y32) return pvd_set_error(PVD_EINVAL, "null FP32 output"); break;

@@ -16,7 +16,15 @@
 
 namespace {
 
-constexpr int kBatch = 8;   // global loads issued per thread before the first is consumed
+#ifndef PVD_LOAD_BATCH
+#define PVD_LOAD_BATCH 16
+#endif
+#ifdef PVD_FFT_WAVES
+#define PVD_FFT_OCC __attribute__((amdgpu_waves_per_eu(PVD_FFT_WAVES)))
+#else
+#define PVD_FFT_OCC
+#endif
+constexpr int kBatch = PVD_LOAD_BATCH;   // global loads issued per thread before the first is consumed
 
 __device__ __forceinline__ int wrap_add(int a, int b, int n) {  // (a + b) mod n, a,b in [0, n)
     int r = a + b;
@@ -78,7 +86,7 @@ struct ZVec {
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
 // holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1).
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[32];
+    __shared__ long long lineoff[2 * PVD_TILE_Z];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
@@ -238,18 +246,40 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __syncthreads();
     fft_lines<false>(s, lnl, TK, ax, tw);
     const int total = Px << lnl;
-    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
-        float2 sv[kBatch];
+    if (v.sym) {
+        // real spectrum, even in kx and ky: read the octant entry of (min(kx, Px-kx), min(y, Py-y))
+        const float *sr = reinterpret_cast<const float *>(v.spec);
+        const int Py = v.ax[1].n;
+        const long long pyh = Py / 2 + 1;
+        const long long row0 = (long long)min(y, Py - y) * Hp + c0;
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float sv[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int e = e0 + u * blockDim.x;
-            const int k = e >> lnl, l = e & (TK - 1);
-            sv[u] = (e < total && l < nc) ? sp[k * plane + l] : make_float2(0.f, 0.f);
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const int k = e >> lnl, l = e & (TK - 1);
+                sv[u] = (e < total && l < nc) ? sr[(long long)min(k, Px - k) * pyh * Hp + row0 + l] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < total) s[e] = cscale(s[e], sv[u]);
+            }
         }
+    } else {
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float2 sv[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int e = e0 + u * blockDim.x;
-            if (e < total) s[e] = cmul(s[e], sv[u]);
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const int k = e >> lnl, l = e & (TK - 1);
+                sv[u] = (e < total && l < nc) ? sp[k * plane + l] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < total) s[e] = cmul(s[e], sv[u]);
+            }
         }
     }
     __syncthreads();
@@ -274,7 +304,7 @@ __device__ __forceinline__ float iz_value(const float2 *s, const pvd_axis_d &az,
 }
 
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[32];
+    __shared__ long long lineoff[2 * PVD_TILE_Z];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
@@ -376,13 +406,13 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
 
 // single volume: descriptor by value (kernarg segment, scalar loads)
 template <int PASS>
-__global__ __launch_bounds__(1024) void pvd_fft_pass(pvd_vol_d v) {
+__global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
     run_pass<PASS>(v, blockIdx.x);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
 template <int PASS>
-__global__ __launch_bounds__(1024) void pvd_fft_pass_batch(pvd_batch_args a) {
+__global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch_args a) {
     const int b = blockIdx.x;
     int lo = 0, hi = a.nvol - 1;
     while (lo < hi) {   // last volume with tile_base <= b

@@ -14,6 +14,18 @@
 //   off   output o reads FFT index (o + off) mod n
 //   foldn outputs o < foldn also add FFT index (o + N + off) mod n
 //         (reference-mode circular wrap when the FFT is padded, P != N)
+// Tile targets: complex line pairs per tile of the z passes (each pair is two real
+// lines), and kz columns per tile of the y and x passes.  Powers of two.
+#ifndef PVD_TILE_Z
+#define PVD_TILE_Z 16
+#endif
+#ifndef PVD_TILE_Y
+#define PVD_TILE_Y 16
+#endif
+#ifndef PVD_TILE_X
+#define PVD_TILE_X 16
+#endif
+
 struct pvd_axis_d {
     int n;
     int N;
@@ -39,6 +51,8 @@ struct pvd_vol_d {
     int ltky;                         // log2 of columns per y-pass tile
     int ltkx;                         // log2 of columns per x-pass tile
     int vec_io;                       // 1: act/dens/out are 16-byte aligned (float4 access)
+    int sym;                          // 1: spec holds the real octant [Px/2+1][Py/2+1][Hp] (FP32)
+    int pad3_;
     const float *act[PVD_MAX_TIMEPOINTS];
     float w[PVD_MAX_TIMEPOINTS];
     const float *dens;
@@ -46,7 +60,7 @@ struct pvd_vol_d {
     float scale;
     int pad2_;
     float2 *work;                     // [Nx][Py][Hp] complex workspace
-    const float2 *spec;               // [Px][Py][Hp] kernel spectrum (1/P folded in)
+    const float2 *spec;               // [Px][Py][Hp] kernel spectrum (1/P folded in); see sym
     int tile_base[PVD_NPASS_FFT];     // first grid tile of this volume per pass (batch)
     int tiles[PVD_NPASS_FFT];         // tiles of this volume per pass
 };

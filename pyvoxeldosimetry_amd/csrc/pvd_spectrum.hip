@@ -71,6 +71,7 @@ struct pvd_spec_args {
     int K[3];
     int H;
     int Hp;   // row pitch of the FP32 spectrum
+    int sym;  // 1: write only the real part, octant kx <= Px/2, ky <= Py/2, as FP32 scalars
 };
 
 __device__ __forceinline__ int spec_pos(const pvd_spec_axis &ax, int m) {
@@ -153,6 +154,8 @@ __global__ __launch_bounds__(256) void pvd_spec_x(pvd_spec_args p, const double2
          idx += (long long)gridDim.x * blockDim.x) {
         const long long c = idx % plane;
         const int kx = (int)(idx / plane);
+        const long long ky = c / p.H, kz = c - ky * p.H;
+        if (p.sym && (kx > ax.P / 2 || ky > p.a[1].P / 2)) continue;
         double2 acc = make_double2(0.0, 0.0);
         for (int a = 0; a < ax.u; ++a) {
             const double2 t = T2[(long long)a * plane + c];
@@ -161,8 +164,12 @@ __global__ __launch_bounds__(256) void pvd_spec_x(pvd_spec_args p, const double2
             acc.x += r.x;
             acc.y += r.y;
         }
-        const long long ky = c / p.H, kz = c - ky * p.H;
-        S[((long long)kx * p.a[1].P + ky) * p.Hp + kz] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
+        if (p.sym) {
+            float *sr = reinterpret_cast<float *>(S);
+            sr[((long long)kx * (p.a[1].P / 2 + 1) + ky) * p.Hp + kz] = (float)(acc.x * norm);
+        } else {
+            S[((long long)kx * p.a[1].P + ky) * p.Hp + kz] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
+        }
     }
 }
 
@@ -172,9 +179,11 @@ static int grid_for(long long total, int block) {
     return (int)(g > 0 ? g : 1);
 }
 
-// T1/T2 are caller-provided FP64 scratch of ux*uy*H and ux*Py*H double2.
+// T1/T2 are caller-provided FP64 scratch of ux*uy*H and ux*Py*H double2.  With
+// sym, S receives the real octant [Px/2+1][Py/2+1][Hp] as FP32 scalars.
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
-                               int Hp, const double *kernel, double2 *T1, double2 *T2, float2 *S, hipStream_t st) {
+                               int Hp, int sym, const double *kernel, double2 *T1, double2 *T2, float2 *S,
+                               hipStream_t st) {
     pvd_spec_args p;
     for (int i = 0; i < 3; ++i) {
         p.a[i].P = P[i]; p.a[i].lo = lo[i]; p.a[i].u = u[i]; p.a[i].s = s[i];
@@ -182,6 +191,7 @@ hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], 
     }
     p.H = P[2] / 2 + 1;
     p.Hp = Hp;
+    p.sym = sym;
     const int block = 256;
     hipLaunchKernelGGL(pvd_spec_z, dim3(grid_for((long long)u[0] * u[1] * p.H, block)), dim3(block),
                        sizeof(double2) * P[2], st, p, kernel, T1);

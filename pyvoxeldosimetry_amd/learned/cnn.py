@@ -88,6 +88,29 @@ class ResidualCNN:
         self.v.copy_(sd["v"].to(self.device))
         self.step = int(sd["step"])
 
+    def save_weights(self, path: str, metadata=None) -> None:
+        """FP32 master weights only, as safetensors (data, no code)."""
+        from safetensors.torch import save_file
+        meta = {k: str(v) for k, v in (metadata or {}).items()}
+        meta["layout"] = "flat: per layer w[27][32][cin] then b[32]; cin = " + ",".join(map(str, CIN))
+        save_file({"master": self.master.detach().cpu().contiguous()}, path, metadata=meta)
+
+    def load_weights(self, path: str) -> dict:
+        """Load weights written by save_weights; resets the optimiser state. Returns the metadata."""
+        import torch
+        from safetensors import safe_open
+        with safe_open(path, framework="pt") as f:
+            meta = dict(f.metadata() or {})
+            master = f.get_tensor("master")
+        if master.dtype != torch.float32 or master.numel() != self.n_params:
+            raise ValueError(f"{path}: expected {self.n_params} FP32 parameters, got {master.numel()} {master.dtype}")
+        self.master.copy_(master.to(self.device))
+        self.bf16.copy_(self.master.to(torch.bfloat16))
+        self.m.zero_()
+        self.v.zero_()
+        self.step = 0
+        return meta
+
     # ------------------------------------------------------------------ workspace
     def _workspace(self, shape):
         import torch
@@ -122,13 +145,15 @@ class ResidualCNN:
                   "pvd_cnn_conv_fwd")
 
     # ------------------------------------------------------------------ forward / backward
-    def forward(self, x, shape):
-        """x: BF16 [n, 16] inputs; returns the FP32 correction r [n] (workspace tensor)."""
+    def forward(self, x, shape, keep_activations: bool = True):
+        """x: BF16 [n, 16] inputs; returns the FP32 correction r [n] (workspace tensor).
+        ``keep_activations=False`` (inference) skips the ReLU outputs backward needs."""
         ws = self._workspace(shape)
         h = ws["h"]
         self._conv(x, 16, 0, nat.PVD_CNN_EPI_RELU, shape, y=h[0])
         for l in range(1, 5):
-            self._conv(h[l - 1], 32, l, nat.PVD_CNN_EPI_RESID_RELU, shape, y=h[l], act=ws["act"][l], resid=h[l - 1])
+            self._conv(h[l - 1], 32, l, nat.PVD_CNN_EPI_RESID_RELU, shape, y=h[l],
+                       act=ws["act"][l] if keep_activations else None, resid=h[l - 1])
         self._conv(h[4], 32, 5, nat.PVD_CNN_EPI_LINEAR_F32, shape, y32=ws["r"])
         return ws["r"]
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 import time
 from dataclasses import dataclass
 
@@ -22,6 +23,10 @@ from .. import _native as nat
 from ..plan import ConvPlan
 from .cnn import ResidualCNN
 from .mc import MonteCarlo
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+PRETRAINED = {("Y90", 2.0): "cnn_y90_2mm.safetensors"}
+EVAL_SEEDS_TEST = range(20000, 20006)      # held-out phantoms (training used 10000..10299)
 
 
 def synth_phantom(seed: int, shape=(128, 128, 128), device=None):
@@ -86,6 +91,20 @@ class LearnedCorrector:
         self._plans = {}
         self._bufs = {}
 
+    @classmethod
+    def pretrained(cls, nuclide: str = "Y90", voxel_mm: float = 2.0, **kw) -> "LearnedCorrector":
+        """Corrector with the weights shipped in ``learned/weights`` (trained by
+        ``learned/train.py`` on this package's Monte Carlo; see the file's metadata)."""
+        name = PRETRAINED.get((nuclide, float(voxel_mm)))
+        if name is None:
+            raise ValueError(f"no pretrained weights for {nuclide} at {voxel_mm} mm; "
+                             f"available: {sorted(PRETRAINED)}")
+        meta = dict(bound=0.9)
+        meta.update(kw)
+        c = cls(nuclide, voxel_mm=voxel_mm, **meta)
+        c.weights_meta = c.net.load_weights(os.path.join(WEIGHTS_DIR, name))
+        return c
+
     def _stream(self):
         return ctypes.c_void_p(nat.current_stream_handle(self.device))
 
@@ -148,7 +167,7 @@ class LearnedCorrector:
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
         if r_override is None:
             x, _ = self.pack(act, rho, dp)
-            r = self.net.forward(x, shape)
+            r = self.net.forward(x, shape, keep_activations=False)
         else:
             r = r_override.reshape(-1).contiguous()
         out = torch.empty(shape, dtype=torch.float32, device=self.device)
@@ -177,7 +196,8 @@ def voxel_errors(dose, ref, region_frac: float = 0.1):
 def evaluate(corr: LearnedCorrector, seeds, shape=(128, 128, 128), histories: int = 128, tol: float = 0.02):
     """Plain and CNN-corrected dose vs Monte Carlo on held-out phantoms."""
     import torch
-    stats = {"plain_within": [], "cnn_within": [], "plain_mean": [], "cnn_mean": [], "fallbacks": 0}
+    stats = {"plain_within": [], "cnn_within": [], "plain_mean": [], "cnn_mean": [], "plain_p90": [], "cnn_p90": [],
+             "fallbacks": 0}
     for s in seeds:
         act, rho = synth_phantom(s, shape, corr.device)
         ref = corr.mc_dose(act, rho, histories, seed=s)
@@ -188,6 +208,8 @@ def evaluate(corr: LearnedCorrector, seeds, shape=(128, 128, 128), histories: in
         stats["cnn_within"].append(float((rc <= tol).float().mean().item()))
         stats["plain_mean"].append(float(rp.mean().item()))
         stats["cnn_mean"].append(float(rc.mean().item()))
+        stats["plain_p90"].append(float(torch.quantile(rp, 0.9).item()))
+        stats["cnn_p90"].append(float(torch.quantile(rc, 0.9).item()))
         stats["fallbacks"] += int(c.fallback)
     out = {k: (float(np.mean(v)) if isinstance(v, list) else v) for k, v in stats.items()}
     out["n"] = len(list(seeds)) if not isinstance(seeds, range) else len(seeds)
@@ -212,7 +234,7 @@ def time_per_volume(corr: LearnedCorrector, shape=(128, 128, 128), reps: int = 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        corr.net.forward(x, tuple(shape))
+        corr.net.forward(x, tuple(shape), keep_activations=False)
     torch.cuda.synchronize()
     t_net = (time.perf_counter() - t0) * 1e3 / reps
     t0 = time.perf_counter()

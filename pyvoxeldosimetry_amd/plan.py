@@ -32,7 +32,7 @@ class ConvPlan:
 
     def __init__(self, shape: Sequence[int], kernel_shape: Sequence[int], mode: str = "reference",
                  algo: str = "auto", fft_n: Optional[Sequence[int]] = None, profile: bool = False,
-                 device=None):
+                 device=None, general_spectrum: bool = False):
         import torch
         nat.require_device()
         self.lib = nat.load()
@@ -52,7 +52,8 @@ class ConvPlan:
         prm.mode = nat.MODES[mode]
         prm.algo = nat.ALGOS[algo]
         prm.fft_n[:] = tuple(int(x) for x in fft_n) if fft_n else (0, 0, 0)
-        prm.flags = nat.PVD_FLAG_PROFILE if profile else 0
+        prm.flags = (nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum
+                                                                 else 0)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             nat.check(self.lib.pvd_plan_create(ctypes.byref(h), ctypes.byref(prm)), "pvd_plan_create")
@@ -64,12 +65,14 @@ class ConvPlan:
         self.kernel_used = tuple(info.kernel_used)
         self.n_passes = info.n_passes
         self.workspace_bytes = int(info.workspace_bytes)
+        self.symmetric_spectrum = False
         self._kernel_dev = None
 
     # ------------------------------------------------------------------
     def info(self) -> dict:
         return {"algo": self.algo, "mode": self.mode, "fft_n": self.fft_n, "kernel_used": self.kernel_used,
-                "n_passes": self.n_passes, "workspace_bytes": self.workspace_bytes}
+                "n_passes": self.n_passes, "workspace_bytes": self.workspace_bytes,
+                "symmetric_spectrum": self.symmetric_spectrum}
 
     def set_kernel(self, kernel) -> None:
         """FP64 dose kernel (torch tensor on the device or anything numpy accepts)."""
@@ -86,6 +89,9 @@ class ConvPlan:
                                                    ctypes.c_void_p(nat.current_stream_handle(self.device))),
                       "pvd_plan_set_kernel")
         self._kernel_dev = k
+        info = nat.PlanInfo()
+        nat.check(self.lib.pvd_plan_info_get(self._h, ctypes.byref(info)), "pvd_plan_info_get")
+        self.symmetric_spectrum = bool(info.symmetric_spectrum)
 
     def _prep(self, density, out):
         import torch

@@ -147,6 +147,31 @@ def test_ct_config_linear_mode():
     _say(f"ct linear max_rel={m['max_rel']:.2e} fft_n={plan.fft_n}")
 
 
+@pytest.mark.parametrize("mode", ["reference", "linear"])
+def test_symmetric_and_general_spectra_agree(mode):
+    """Real-octant spectrum (odd symmetric kernel) vs the full complex spectrum vs the oracle,
+    on a smooth size (reference: P = N, output roll) and a prime size (padded + fold)."""
+    from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
+    _say("symmetric spectrum", mode)
+    rng = np.random.default_rng(21)
+    k = Lu177KernelGenerator("water").generate_kernel_device(1.0, (9, 9, 9))
+    for shape in ((40, 36, 30), (29, 31, 23)):
+        a = rng.random(shape).astype(np.float32)
+        oracle_fn = conv.ref_conv if mode == "reference" else conv.linear_same
+        exp = oracle_fn(a.astype(np.float64), k.cpu().numpy())
+        outs = []
+        for general in (False, True):
+            plan = ConvPlan(shape, (9, 9, 9), mode=mode, algo="fft", general_spectrum=general)
+            plan.set_kernel(k)
+            assert plan.symmetric_spectrum == (not general)
+            outs.append(plan.dose(dev_tensor(a)).cpu().numpy())
+            assert_parity(outs[-1], exp, what=f"{mode} {shape} general={general}")
+    # non-symmetric kernels keep the general spectrum
+    plan = ConvPlan((20, 20, 20), (5, 5, 5), algo="fft")
+    plan.set_kernel(rng.random((5, 5, 5)))
+    assert not plan.symmetric_spectrum
+
+
 def test_nonsmooth_reference_fold_against_bruteforce_c():
     """Prime-sized grid: padded FFT + fold must equal circular brute force (independent C)."""
     from oracle import brute

@@ -42,7 +42,7 @@ def test_struct_layouts_match_header():
     from pyvoxeldosimetry_amd import _native as nat
     assert ctypes.sizeof(nat.PlanParams) == 4 * 12
     assert ctypes.sizeof(nat.DpkTerm) == 24
-    assert ctypes.sizeof(nat.PlanInfo) == 8 * 4 + 8
+    assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8      # 9 ints, padding, size_t
 
 
 def test_product_path_fails_loudly_without_gpu():
@@ -148,3 +148,26 @@ def test_autotune_candidate_lengths_do_not_alias():
                 assert p == n or p >= n + min(k, n) - 1
             else:
                 assert p >= n + k // 2
+
+
+def test_shipped_cnn_weights_are_well_formed():
+    """The pretrained weights are data only (safetensors), of the network's size,
+    finite, and zero where the architecture fixes them (layer 0 reads 3 input
+    channels, layer 5 writes one)."""
+    from safetensors import safe_open
+
+    from pyvoxeldosimetry_amd.learned.cnn import CIN
+    from pyvoxeldosimetry_amd.learned.pipeline import PRETRAINED, WEIGHTS_DIR
+    for name in PRETRAINED.values():
+        with safe_open(os.path.join(WEIGHTS_DIR, name), framework="np") as f:
+            meta = f.metadata()
+            w = f.get_tensor("master")
+        assert w.dtype == np.float32
+        assert w.size == sum(27 * 32 * c + 32 for c in CIN)
+        assert np.isfinite(w).all()
+        w0 = w[:27 * 32 * 16].reshape(27, 32, 16)
+        assert not w0[:, :, 3:].any() and w0[:, :, :3].any()
+        off5 = sum(27 * 32 * c + 32 for c in CIN[:5])
+        w5 = w[off5:off5 + 27 * 32 * 32].reshape(27, 32, 32)
+        assert not w5[:, 1:, :].any() and w5[:, 0, :].any()
+        assert {"nuclide", "voxel_mm", "train_seeds", "command"} <= set(meta)
