@@ -269,13 +269,16 @@ int fft_setup(pvd_plan *p) {
     const int Nx = p->N[0], Ny = p->N[1];
     const int Px = p->P[0], Py = p->P[1], Pz = p->P[2];
     v.H = Pz / 2 + 1;
-    v.Hp = round_up(v.H, 16);
     int lz, bz; size_t ldsz;
     pick_tile(Pz, PVD_TILE_Z, 1, &lz, &bz, &ldsz);
     int tky, by; size_t ldsy;
     pick_tile(Py, PVD_TILE_Y, 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
     pick_tile(Px, PVD_TILE_X, 0, &tkx, &bx, &ldsx);
+    // the y and x passes move 16-byte column pairs and read whole tiles of columns
+    if (tky < 2 || tkx < 2) return fail(PVD_ENOTSUP, "FFT length too large for a two-column tile");
+    // rows padded to 128 bytes and to whole column tiles
+    v.Hp = round_up(v.H, std::max(16, std::max(tky, tkx)));
     v.llz = ilog2(lz);
     v.ltky = ilog2(tky);
     v.ltkx = ilog2(tkx);

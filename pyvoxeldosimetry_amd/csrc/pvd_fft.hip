@@ -37,23 +37,55 @@ __device__ __forceinline__ void load_twiddles(float2 *tw, const pvd_axis_d &ax) 
 
 // Load rows [0, rows_in) x columns [0, nc) of a strided tile into LDS
 // s[k*nl + l] (zero elsewhere up to n rows); row k starts at g + k*pitch.
+// Moves 16-byte column pairs: g and pitch are even (in complex elements) and
+// every row holds 2^lnl readable columns (rows are padded to whole tiles), so
+// columns >= nc are read and then replaced by zero.
 __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const float2 *__restrict__ g, long long pitch,
                                               int n, int rows_in, int lnl, int nc) {
-    const int total = n << lnl;
-    const int mask = (1 << lnl) - 1;
+    const int lp = lnl - 1;
+    const int total = n << lp;
+    const int pmask = (1 << lp) - 1;
+    float4 *s4 = reinterpret_cast<float4 *>(s);
     for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
-        float2 v[kBatch];
+        float4 v[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * blockDim.x;
-            const int k = e >> lnl, l = e & mask;
-            v[u] = (e < total && k < rows_in && l < nc) ? g[k * pitch + l] : make_float2(0.f, 0.f);
+            const int k = e >> lp, c = (e & pmask) << 1;
+            v[u] = (e < total && k < rows_in) ? *reinterpret_cast<const float4 *>(g + k * pitch + c)
+                                              : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * blockDim.x;
-            if (e < total) s[e] = v[u];
+            if (e < total) {
+                const int c = (e & pmask) << 1;
+                float4 x = v[u];
+                if (c >= nc) { x.x = 0.f; x.y = 0.f; }
+                if (c + 1 >= nc) { x.z = 0.f; x.w = 0.f; }
+                s4[e] = x;
+            }
         }
+    }
+}
+
+// Store rows o in [0, rows_out) of a column tile from LDS, whole 16-byte column
+// pairs (columns >= nc land in row padding): row o comes from LDS row
+// (o + off) mod P, plus row (o + rows_out + off) mod P when o < foldn.
+__device__ __forceinline__ void store_col_tile(float2 *__restrict__ g, long long pitch, const float2 *__restrict__ s,
+                                               int rows_out, int lnl, int P, int off, int foldn) {
+    const int lp = lnl - 1;
+    const int total = rows_out << lp;
+    const int pmask = (1 << lp) - 1;
+    const float4 *s4 = reinterpret_cast<const float4 *>(s);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int o = e >> lp, cp = e & pmask;
+        float4 val = s4[(wrap_add(o, off, P) << lp) + cp];
+        if (o < foldn) {
+            const float4 f = s4[(((o + rows_out + off) % P) << lp) + cp];
+            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
+        }
+        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
     }
 }
 
@@ -209,21 +241,8 @@ __device__ void pass_y(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
     __syncthreads();
     fft_lines<INV>(s, lnl, TK, ay, tw);
-    if (!INV) {
-        for (int e = threadIdx.x; e < (Py << lnl); e += blockDim.x) {
-            const int k = e >> lnl, l = e & (TK - 1);
-            if (l < nc) g[(long long)k * Hp + l] = s[e];
-        }
-    } else {
-        for (int e = threadIdx.x; e < (Ny << lnl); e += blockDim.x) {
-            const int o = e >> lnl, l = e & (TK - 1);
-            if (l < nc) {
-                float2 val = s[(wrap_add(o, ay.off, Py) << lnl) + l];
-                if (o < ay.foldn) val = cadd(val, s[(((o + Ny + ay.off) % Py) << lnl) + l]);
-                g[(long long)o * Hp + l] = val;
-            }
-        }
-    }
+    if (!INV) store_col_tile(g, Hp, s, Py, lnl, Py, 0, 0);
+    else store_col_tile(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn);
 }
 
 // ---------------------------------------------------------------------------------
@@ -284,14 +303,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     }
     __syncthreads();
     fft_lines<true>(s, lnl, TK, ax, tw);
-    for (int e = threadIdx.x; e < (Nx << lnl); e += blockDim.x) {
-        const int o = e >> lnl, l = e & (TK - 1);
-        if (l < nc) {
-            float2 val = s[(wrap_add(o, ax.off, Px) << lnl) + l];
-            if (o < ax.foldn) val = cadd(val, s[(((o + Nx + ax.off) % Px) << lnl) + l]);
-            g[o * plane + l] = val;
-        }
-    }
+    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn);
 }
 
 // ---------------------------------------------------------------------------------
@@ -359,22 +371,33 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps
         const ZVec zv(nreal, Nz);
         const float inv = 1.0f / (float)zv.nreal16;
-        for (int i = threadIdx.x; i < zv.items; i += blockDim.x) {
-            int ln, f;
-            if (!zv.decode(i, inv, ln, f)) continue;
-            const long long off = obase + (long long)ln * Nz;
-            float4 d = make_float4(1.f, 1.f, 1.f, 1.f);
-            if (has_dens) d = reinterpret_cast<const float4 *>(v.dens + off)[f];
-            const int o = 4 * f;
-            float4 res = make_float4(iz_value(s, az, ld, o, ln) * sc, iz_value(s, az, ld, o + 1, ln) * sc,
-                                     iz_value(s, az, ld, o + 2, ln) * sc, iz_value(s, az, ld, o + 3, ln) * sc);
-            if (has_dens) {
-                res.x *= density_factor(d.x);
-                res.y *= density_factor(d.y);
-                res.z *= density_factor(d.z);
-                res.w *= density_factor(d.w);
+        constexpr int kEpi = 4;    // density loads in flight per thread
+        for (int i0 = threadIdx.x; i0 < zv.items; i0 += kEpi * blockDim.x) {
+            float4 d[kEpi];
+            int lnv[kEpi], fv[kEpi];
+#pragma unroll
+            for (int u = 0; u < kEpi; ++u) {
+                int ln, f;
+                const bool ok = zv.decode(i0 + u * blockDim.x, inv, ln, f);
+                lnv[u] = ok ? ln : -1;
+                fv[u] = f;
+                d[u] = make_float4(1.f, 1.f, 1.f, 1.f);
+                if (ok && has_dens) d[u] = reinterpret_cast<const float4 *>(v.dens + obase + (long long)ln * Nz)[f];
             }
-            reinterpret_cast<float4 *>(v.out + off)[f] = res;
+#pragma unroll
+            for (int u = 0; u < kEpi; ++u) {
+                if (lnv[u] < 0) continue;
+                const int ln = lnv[u], o = 4 * fv[u];
+                float4 res = make_float4(iz_value(s, az, ld, o, ln) * sc, iz_value(s, az, ld, o + 1, ln) * sc,
+                                         iz_value(s, az, ld, o + 2, ln) * sc, iz_value(s, az, ld, o + 3, ln) * sc);
+                if (has_dens) {
+                    res.x *= density_factor(d[u].x);
+                    res.y *= density_factor(d[u].y);
+                    res.z *= density_factor(d[u].z);
+                    res.w *= density_factor(d[u].w);
+                }
+                reinterpret_cast<float4 *>(v.out + obase + (long long)ln * Nz)[fv[u]] = res;
+            }
         }
     } else {
         const int nout = nreal * Nz;

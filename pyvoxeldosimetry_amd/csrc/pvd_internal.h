@@ -304,4 +304,5 @@ int pvd_set_error(int code, const char *fmt, ...);
 int pvd_hip_error(hipError_t e, const char *what);
 
 // rho_w / rho with rho_w = 1 g/cc; 0 where rho <= 0 (declared epilogue).
-__device__ __forceinline__ float density_factor(float rho) { return rho > 0.0f ? 1.0f / rho : 0.0f; }
+// The hardware reciprocal (within 1 ulp) replaces the IEEE division sequence.
+__device__ __forceinline__ float density_factor(float rho) { return rho > 0.0f ? __builtin_amdgcn_rcpf(rho) : 0.0f; }
