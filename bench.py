@@ -89,9 +89,12 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 256 MiB (reference for rocprof byte counters)")
-    ap.add_argument("--config", default="ct", choices=["ct", "batch", "torso"],
+    ap.add_argument("--config", default="ct", choices=["ct", "batch", "torso", "tissue", "cnn", "tac"],
                     help="ct: 256x256x200 single volume (default, the headline metric); batch: 48 uneven "
-                         "phantoms x 4 timepoints in one graph; torso: 512x512x400, 61^3, 6 timepoints")
+                         "phantoms x 4 timepoints in one graph; torso: 512x512x400, 61^3, 6 timepoints; "
+                         "tissue: HU -> metal fill, class masks and density of a 512x512x400 CT; "
+                         "cnn: CNN-corrected Y-90 dose of a 128^3 phantom (time, MFMA use, error vs Monte Carlo); "
+                         "tac: voxelwise mono-exponential TAC fit + integral of 256x256x200 voxels x 4 timepoints")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -118,6 +121,12 @@ def main():
         return bench_batch(args)
     if args.config == "torso":
         return bench_torso(args)
+    if args.config == "tissue":
+        return bench_tissue(args)
+    if args.config == "cnn":
+        return bench_cnn(args)
+    if args.config == "tac":
+        return bench_tac(args)
 
     from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
     from pyvoxeldosimetry_amd.phantoms import ct_phantom
@@ -297,6 +306,169 @@ def bench_torso(args):
                                "sample": f"one reference np.fft conv of the 512x512x400 map ({one:.2f} s); the "
                                          f"reference repeats it per timepoint"}
         res["speedup_vs_cpu_reference"] = round(res["value"] / cpu_v, 1)
+    print(json.dumps(res), flush=True)
+
+
+def bench_cnn(args):
+    """Learned heterogeneity correction: plain FFT dose + residual CNN (BF16 MFMA) of a
+    128^3 Y-90 phantom at 2 mm; error of the shipped network against held-out Monte Carlo."""
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd.learned.pipeline import EVAL_SEEDS_TEST, LearnedCorrector, evaluate, synth_phantom
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    corr = LearnedCorrector.pretrained("Y90", 2.0)
+    shape = (128, 128, 128)
+    act, rho = synth_phantom(123, shape, dev)
+    ms_plain = _timed(lambda: corr.plain(act, rho), args.steps, args.warmup)
+    ms_corr = _timed(lambda: corr.correct(act, rho), max(1, args.steps // 5), args.warmup)
+    x, _ = corr.pack(act, rho, corr.plain(act, rho))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    corr.net.forward(x, shape, keep_activations=False)
+    ev0.record()
+    for _ in range(reps):
+        corr.net.forward(x, shape, keep_activations=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms_net = ev0.elapsed_time(ev1) / reps
+    n = int(np.prod(shape))
+    alg = 2.0 * 27 * (3 * 32 + 4 * 32 * 32 + 32 * 1) * n          # useful MACs x 2 (3 in, 32 hidden, 1 out)
+    executed = 2.0 * 27 * (16 * 32 + 4 * 32 * 32 + 32 * 32) * n    # as issued (16/32-channel MFMA tiles)
+    peak = 2500.0
+    ev = evaluate(corr, EVAL_SEEDS_TEST, (64, 64, 64), histories=8192)
+    res = {"metric": "CNN-corrected ms/vol (Y-90, 128^3 @2 mm) and error vs held-out Monte Carlo",
+           "value": round(ms_corr, 4), "unit": "ms/vol", "n_gpus": 1, "steps": max(1, args.steps // 5),
+           "warmup": args.warmup, "ms_per_step": round(ms_corr, 4), "higher_is_better": False, "dtype": "bf16",
+           "data": "synthetic (synth_phantom seed 123; Monte Carlo targets of held-out seeds 20000..20005)",
+           "config": {"workload": "plain FFT dose (linear, 13^3 MC kernel) + 6-layer 3^3 residual CNN "
+                                  "(32 ch, BF16 MFMA, FP32 accumulate) + bounded correction, 128^3",
+                      "plain_ms": round(ms_plain, 4), "cnn_forward_ms": round(ms_net, 4),
+                      "ratio_corrected_over_plain": round(ms_corr / ms_plain, 2)},
+           "roofline": {"bound": "mfma", "achieved": round(alg / (ms_net * 1e-3) / 1e12, 1), "peak": peak,
+                        "unit": "TFLOP/s", "frac": round(alg / (ms_net * 1e-3) / 1e12 / peak, 4),
+                        "executed_TFLOPs": round(executed / (ms_net * 1e-3) / 1e12, 1),
+                        "executed_frac": round(executed / (ms_net * 1e-3) / 1e12 / peak, 4),
+                        "traffic": None, "kernel": "cnn_conv_fwd (6 launches)", "kernel_ms": round(ms_net, 5),
+                        "algorithmic_flops": alg},
+           "error_vs_monte_carlo_64cubed": ev}
+    if not args.no_cpu:
+        from oracle import conv
+        a64 = act.cpu().numpy().astype(np.float64)
+        k64 = corr.kernel.cpu().numpy()
+        conv.linear_same(a64, k64)
+        t0 = time.perf_counter()
+        conv.linear_same(a64, k64)
+        t_cpu = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(t_cpu * 1e3, 2), "unit": "ms/vol", "cores": 1, "kind": "port",
+                               "sample": "plain convolution of the same 128^3 map on the host (np.fft complex128, "
+                                         "1 core); the reference has no learned correction"}
+    print(json.dumps(res), flush=True)
+
+
+def bench_tac(args):
+    """Voxelwise A0 exp(-lam t) fit (FP64 Levenberg-Marquardt per voxel) and the
+    A0/lam (1 - exp(-lam 100 T1/2)) integral of a 256x256x200 x 4-timepoint series."""
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd.phantoms import BATCH_TIMEPOINTS_H
+    from pyvoxeldosimetry_amd.time_integration import TimeCurveFitting
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    rng = np.random.default_rng(12)
+    shape = (256, 256, 200)
+    n = int(np.prod(shape))
+    t = np.array(BATCH_TIMEPOINTS_H, dtype=np.float64)
+    a0 = torch.from_numpy(rng.uniform(1e3, 1e5, n).astype(np.float32)).to(dev)
+    lam = torch.from_numpy(rng.uniform(0.004, 0.06, n).astype(np.float32)).to(dev)
+    maps = [(a0 * torch.exp(-lam * float(ti)) * (1.0 + 0.03 * torch.randn(n, device=dev))).reshape(shape)
+            for ti in t]
+    tcf = TimeCurveFitting(64.1)
+    run = lambda: tcf._fit(t, maps, 0, [0.0, tcf.decay_constant], None, False, 200,  # noqa: E731
+                           [0.0, tcf.decay_constant], 0, t_int=100 * 64.1, p0_from_data=True)
+    ms = _timed(run, max(1, args.steps // 50), min(args.warmup, 2))
+    params, auc, status, _ = run()
+    torch.cuda.synchronize()
+    res = {"metric": "Gvoxel/s voxelwise mono-exponential TAC fit + integral, 256x256x200 x 4 timepoints",
+           "value": round(n / (ms * 1e-3) / 1e9, 4), "unit": "Gvoxel/s", "n_gpus": 1,
+           "steps": max(1, args.steps // 50), "warmup": min(args.warmup, 2), "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic (A0 U[1e3,1e5], lam U[0.004,0.06]/h, 3% noise)",
+           "config": {"workload": "TimeCurveFitting.fit_time_activity_curve, 13.1 M voxels, T=4 at 4/24/72/144 h",
+                      "converged_fraction": float(status.float().mean().item())},
+           "roofline": {"bound": "hbm", "achieved": round(n * (16 + 12) / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(n * 28 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "tac_fit_kernel", "note": "FP64 VALU-bound (exp and 2x2 solves per iteration); "
+                                                           "28 B/voxel moved"}}
+    if not args.no_cpu:
+        from oracle import tac
+        m = 3000
+        y = np.stack([mp.reshape(-1)[:m].cpu().numpy().astype(np.float64) for mp in maps], axis=1)
+        t0 = time.perf_counter()
+        pe, acce = tac.fit_basic(t, y, 64.1)
+        t_cpu = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / t_cpu / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": "port",
+                               "sample": f"reference loop (scipy curve_fit per voxel) on the first {m} voxels, "
+                                         f"{t_cpu:.2f} s"}
+        res["speedup_vs_cpu_reference"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+        pg = params[:, :m].cpu().numpy().astype(np.float64)
+        res["max_rel_param_err_vs_curve_fit"] = float(np.max(np.abs(pg - pe) / np.abs(pe)))
+    print(json.dumps(res), flush=True)
+
+
+def bench_tissue(args):
+    """HU -> (metal fill, 5 class masks as bits, density) of a 512x512x400 torso CT in one pass."""
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd.phantoms import ct_hu_phantom
+    from pyvoxeldosimetry_amd.tissue import HU_MATERIALS, TissueComposition
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ct_np = ct_hu_phantom(seed=6)
+    ct = torch.from_numpy(ct_np).to(dev)
+    tc = TissueComposition()
+    ms = _timed(lambda: tc.map(ct), args.steps, args.warmup)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    reps = 20
+    for _ in range(reps):
+        corr, bits, dens = tc.map(ct)
+    ev1.record()
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    n = ct.numel()
+    alg = 13 * n                 # 4 B HU read; 4 B corrected HU, 1 B class bits, 4 B density written
+    achieved = alg / (k_ms * 1e-3) / 1e9
+    res = {"metric": "Gvoxel/s HU -> metal fill + tissue classes + density, 512x512x400 CT",
+           "value": round(n / (ms * 1e-3) / 1e9, 3), "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic (ct_hu_phantom seed 6, two 3000 HU implants)",
+           "config": {"workload": "tissue map 512x512x400 f32 HU, metal fill sigma 1 (r=4), 5 classes, "
+                                  "16-material density", "metal_voxels": int((ct_np > 2000).sum())},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "tissue_map4",
+                        "kernel_ms": round(k_ms, 5), "algorithmic_bytes": alg}}
+    if not args.no_cpu:
+        from oracle import tissue as ot
+        slab = ct_np[:, :, :100]
+        t0 = time.perf_counter()
+        comp = ot.calculate_composition(slab)
+        t_cpu = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": slab.size / t_cpu / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": "port",
+                               "sample": f"reference calculate_composition (scipy gaussian_filter + 5 masks, "
+                                         f"float64) on a 512x512x100 slab, {t_cpu:.2f} s"}
+        res["speedup_vs_cpu_reference"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+        exp = ot.handle_artifacts(ct_np)
+        got = corr.cpu().numpy()
+        res["max_abs_err_hu"] = float(np.nanmax(np.abs(got.astype(np.float64) - exp)))
+        dexp = ot.density_from_hu(exp, HU_MATERIALS).astype(np.float32)
+        res["density_mismatches"] = int((dens.cpu().numpy() != dexp).sum())
+        del comp
     print(json.dumps(res), flush=True)
 
 

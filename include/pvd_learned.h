@@ -88,15 +88,17 @@ int pvd_cnn_loss(const float *r, const float *dp, const float *dt, const float *
 int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *param_bf16, long long n, float lr,
                  float beta1, float beta2, float eps, int step, void *stream);
 
-/* Network input: x[v] = bf16(activity*sa, density, dose_plain*sd, 0 x 13). */
-int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, float sa, float sd,
-                        void *x, long long n_voxels, void *stream);
+/* Network input: x[v] = bf16(activity/amax, density, dose_plain/dmax, 0 x 13),
+ * amax = *act_max and dmax = *dose_max read on the device (floored at 1e-30),
+ * so the caller needs no host synchronisation for the scales. */
+int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, const float *act_max,
+                        const float *dose_max, void *x, long long n_voxels, void *stream);
 
-/* Corrected dose with fallback: out = dp*(1+r) where |r| <= bound, else dp;
- * exceed[0] += number of voxels with |r| > bound and dp >= region_min (FP32 atomics
- * of whole counts, exact below 2^24 per call). */
-int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, float region_min, float *out,
-                             float *exceed, long long n_voxels, void *stream);
+/* Corrected dose with fallback: out = dp*(1+r) where |r| <= bound, else dp.
+ * counts[0] += voxels with |r| > bound and dp >= region_frac * (*dose_max);
+ * counts[1] += voxels with dp >= region_frac * (*dose_max). */
+int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const float *dose_max, float region_frac,
+                             float *out, unsigned long long *counts, long long n_voxels, void *stream);
 
 /* Sum rows of a [rows][cols] FP32 array in fixed order into out[cols] (+= if accumulate). */
 int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream);

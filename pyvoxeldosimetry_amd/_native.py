@@ -85,9 +85,38 @@ SIGNATURES = {
     "pvd_cnn_adam": (_I, [_P, _P, _P, _P, _P, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, _I, _P]),
     "pvd_reduce_rows": (_I, [_P, _I, _I, _P, _I, _P]),
-    "pvd_cnn_pack_inputs": (_I, [_P, _P, _P, ctypes.c_float, ctypes.c_float, _P, ctypes.c_longlong, _P]),
-    "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, ctypes.c_float, _P, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_pack_inputs": (_I, [_P, _P, _P, _P, _P, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, _P, ctypes.c_float, _P, _P, ctypes.c_longlong, _P]),
+    # include/pvd_tissue.h
+    "pvd_tissue_map": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
+    # include/pvd_tac.h
+    "pvd_tac_n_params": (_I, [_I]),
+    "pvd_tac_fit": (_I, [_P, ctypes.POINTER(_P), ctypes.c_longlong, _P, _P, _P, _P]),
 }
+
+PVD_TAC_MAX_T = 16
+PVD_TAC_MAX_P = 5
+
+
+class TacParams(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int), ("n_times", ctypes.c_int), ("t", ctypes.c_double * PVD_TAC_MAX_T),
+                ("w", ctypes.c_double * PVD_TAC_MAX_T), ("lambda_phys", ctypes.c_double),
+                ("p0", ctypes.c_double * PVD_TAC_MAX_P), ("p0_from_data", ctypes.c_int),
+                ("force_zero_at_t0", ctypes.c_int), ("max_iter", ctypes.c_int), ("tol", ctypes.c_double),
+                ("fail_params", ctypes.c_double * PVD_TAC_MAX_P), ("auc_mode", ctypes.c_int),
+                ("t_int", ctypes.c_double)]
+
+PVD_TISSUE_MAX_CLASSES = 8
+PVD_TISSUE_MAX_MATERIALS = 32
+PVD_TISSUE_MAX_RADIUS = 16
+
+
+class TissueParams(ctypes.Structure):
+    _fields_ = [("n_classes", ctypes.c_int), ("class_lo", ctypes.c_float * PVD_TISSUE_MAX_CLASSES),
+                ("class_hi", ctypes.c_float * PVD_TISSUE_MAX_CLASSES), ("n_materials", ctypes.c_int),
+                ("mat_lo", ctypes.c_float * PVD_TISSUE_MAX_MATERIALS),
+                ("mat_density", ctypes.c_float * PVD_TISSUE_MAX_MATERIALS), ("handle_artifacts", ctypes.c_int),
+                ("metal_hu", ctypes.c_float), ("sigma", ctypes.c_double), ("truncate", ctypes.c_double)]
 
 
 class McParams(ctypes.Structure):

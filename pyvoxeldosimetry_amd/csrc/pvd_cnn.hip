@@ -79,20 +79,37 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
     const int wave = tid >> 6, lane = tid & 63;
     const int wx = wave / FWD_BY, wy = wave % FWD_BY;
     const int m = lane & 31, kh = lane >> 5;
+    // The halo of the next block is fetched into registers while this block's
+    // MFMA loop runs (all loads in flight at once), then written to LDS.
+    constexpr int HALO = HX * HY * HZ * C8;
+    constexpr int PER_T = (HALO + FWD_THREADS - 1) / FWD_THREADS;
+    uint4 pf[PER_T];
+    auto fetch = [&](int b) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int i = tid + u * FWD_THREADS;
+            const int c8 = i % C8, vox = i / C8;
+            const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            pf[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < HALO && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
+                pf[u] = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
+        }
+    };
+    if (blockIdx.x < total) fetch(blockIdx.x);
     for (int b = blockIdx.x; b < total; b += gridDim.x) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
         __syncthreads();   // weights visible / previous block finished with the halo
-        for (int i = tid; i < HX * HY * HZ * C8; i += FWD_THREADS) {
-            const int c8 = i % C8, vox = i / C8;
-            const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
-                v = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
-            *reinterpret_cast<uint4 *>(hl + vox * VS + c8 * 16) = v;
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int i = tid + u * FWD_THREADS;
+            if (i < HALO) *reinterpret_cast<uint4 *>(hl + (i / C8) * VS + (i % C8) * 16) = pf[u];
         }
         __syncthreads();
+        if (b + (int)gridDim.x < total) fetch(b + gridDim.x);
         const int ox = x0 + wx, oy = y0 + wy;
         if (ox >= a.n0 || oy >= a.n1) continue;      // wave-uniform
         f32x16 acc = {};
@@ -322,9 +339,13 @@ __global__ __launch_bounds__(256) void reduce_rows(const float *__restrict__ par
     out[c] = accumulate ? out[c] + s : s;
 }
 
+__device__ __forceinline__ float inv_scale(const float *m) { return 1.0f / fmaxf(*m, 1e-30f); }
+
 __global__ __launch_bounds__(256) void cnn_pack(const float *__restrict__ a, const float *__restrict__ d,
-                                                const float *__restrict__ dp, float sa, float sd,
-                                                uint16_t *__restrict__ x, long long n) {
+                                                const float *__restrict__ dp, const float *__restrict__ amax,
+                                                const float *__restrict__ dmax, uint16_t *__restrict__ x,
+                                                long long n) {
+    const float sa = inv_scale(amax), sd = inv_scale(dmax);
     for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
         const uint32_t c0 = f2bf(a[v] * sa), c1 = f2bf(d[v]), c2 = f2bf(dp[v] * sd);
         uint4 *row = reinterpret_cast<uint4 *>(x + v * 16);
@@ -334,23 +355,34 @@ __global__ __launch_bounds__(256) void cnn_pack(const float *__restrict__ a, con
 }
 
 __global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, const float *__restrict__ r,
-                                                 float bound, float region_min, float *__restrict__ out,
-                                                 float *__restrict__ exceed, long long n) {
-    __shared__ float red[256];
-    float cnt = 0.0f;
+                                                 float bound, const float *__restrict__ dmax, float region_frac,
+                                                 float *__restrict__ out, unsigned long long *__restrict__ counts,
+                                                 long long n) {
+    __shared__ unsigned red[2][256];
+    const float region_min = region_frac * *dmax;
+    unsigned ex = 0, reg = 0;
     for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
-        const float rv = r[v];
+        const float rv = r[v], d = dp[v];
         const bool ok = fabsf(rv) <= bound;      // NaN fails too
-        out[v] = ok ? dp[v] * (1.0f + rv) : dp[v];
-        if (!ok && dp[v] >= region_min) cnt += 1.0f;
+        out[v] = ok ? d * (1.0f + rv) : d;
+        const bool in_region = d >= region_min;
+        ex += (!ok && in_region) ? 1u : 0u;
+        reg += in_region ? 1u : 0u;
     }
-    red[threadIdx.x] = cnt;
+    red[0][threadIdx.x] = ex;
+    red[1][threadIdx.x] = reg;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && red[0] > 0.0f) atomicAdd(exceed, red[0]);
+    if (threadIdx.x == 0) {
+        if (red[0][0]) atomicAdd(&counts[0], (unsigned long long)red[0][0]);
+        if (red[1][0]) atomicAdd(&counts[1], (unsigned long long)red[1][0]);
+    }
 }
 
 int g_num_cus = 0;
@@ -508,24 +540,26 @@ int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *para
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch Adam");
 }
 
-int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, float sa, float sd,
-                        void *x, long long n, void *stream) {
-    if (!activity || !density || !dose_plain || !x || n < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, const float *act_max,
+                        const float *dose_max, void *x, long long n, void *stream) {
+    if (!activity || !density || !dose_plain || !act_max || !dose_max || !x || n < 1)
+        return pvd_set_error(PVD_EINVAL, "bad argument");
     long long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(cnn_pack, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, activity, density,
-                       dose_plain, sa, sd, (uint16_t *)x, n);
+                       dose_plain, act_max, dose_max, (uint16_t *)x, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch input packing");
 }
 
-int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, float region_min, float *out,
-                             float *exceed, long long n, void *stream) {
-    if (!dp || !r || !out || !exceed || n < 1 || !(bound >= 0.0f)) return pvd_set_error(PVD_EINVAL, "bad argument");
+int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const float *dose_max, float region_frac,
+                             float *out, unsigned long long *counts, long long n, void *stream) {
+    if (!dp || !r || !dose_max || !out || !counts || n < 1 || !(bound >= 0.0f) || !(region_frac >= 0.0f))
+        return pvd_set_error(PVD_EINVAL, "bad argument");
     long long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(cnn_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, r, bound, region_min,
-                       out, exceed, n);
+    hipLaunchKernelGGL(cnn_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, r, bound, dose_max,
+                       region_frac, out, counts, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch correction");
 }

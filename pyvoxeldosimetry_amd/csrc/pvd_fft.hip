@@ -38,8 +38,8 @@ __device__ __forceinline__ void load_twiddles(float2 *tw, const pvd_axis_d &ax) 
 // Load rows [0, rows_in) x columns [0, nc) of a strided tile into LDS
 // s[k*nl + l] (zero elsewhere up to n rows); row k starts at g + k*pitch.
 // Moves 16-byte column pairs: g and pitch are even (in complex elements) and
-// every row holds 2^lnl readable columns (rows are padded to whole tiles), so
-// columns >= nc are read and then replaced by zero.
+// rows are padded to whole tiles, so a pair straddling nc is read whole and its
+// upper half replaced by zero; pairs wholly past nc are not read.
 __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const float2 *__restrict__ g, long long pitch,
                                               int n, int rows_in, int lnl, int nc) {
     const int lp = lnl - 1;
@@ -52,8 +52,8 @@ __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const floa
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * blockDim.x;
             const int k = e >> lp, c = (e & pmask) << 1;
-            v[u] = (e < total && k < rows_in) ? *reinterpret_cast<const float4 *>(g + k * pitch + c)
-                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[u] = (e < total && k < rows_in && c < nc) ? *reinterpret_cast<const float4 *>(g + k * pitch + c)
+                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -70,16 +70,18 @@ __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const floa
 }
 
 // Store rows o in [0, rows_out) of a column tile from LDS, whole 16-byte column
-// pairs (columns >= nc land in row padding): row o comes from LDS row
-// (o + off) mod P, plus row (o + rows_out + off) mod P when o < foldn.
+// pairs (the upper half of a pair straddling nc lands in row padding; pairs
+// wholly past nc are skipped): row o comes from LDS row (o + off) mod P, plus
+// row (o + rows_out + off) mod P when o < foldn.
 __device__ __forceinline__ void store_col_tile(float2 *__restrict__ g, long long pitch, const float2 *__restrict__ s,
-                                               int rows_out, int lnl, int P, int off, int foldn) {
+                                               int rows_out, int lnl, int P, int off, int foldn, int nc) {
     const int lp = lnl - 1;
     const int total = rows_out << lp;
     const int pmask = (1 << lp) - 1;
     const float4 *s4 = reinterpret_cast<const float4 *>(s);
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         const int o = e >> lp, cp = e & pmask;
+        if ((cp << 1) >= nc) continue;
         float4 val = s4[(wrap_add(o, off, P) << lp) + cp];
         if (o < foldn) {
             const float4 f = s4[(((o + rows_out + off) % P) << lp) + cp];
@@ -241,8 +243,8 @@ __device__ void pass_y(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
     __syncthreads();
     fft_lines<INV>(s, lnl, TK, ay, tw);
-    if (!INV) store_col_tile(g, Hp, s, Py, lnl, Py, 0, 0);
-    else store_col_tile(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn);
+    if (!INV) store_col_tile(g, Hp, s, Py, lnl, Py, 0, 0, nc);
+    else store_col_tile(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -303,7 +305,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     }
     __syncthreads();
     fft_lines<true>(s, lnl, TK, ax, tw);
-    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn);
+    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
 }
 
 // ---------------------------------------------------------------------------------

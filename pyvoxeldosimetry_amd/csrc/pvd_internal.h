@@ -8,12 +8,6 @@
 
 #define PVD_MAXSTAGE 16
 
-// One FFT axis of one volume.
-//   n     FFT length P (7-smooth)
-//   N     logical length (activity grid extent)
-//   off   output o reads FFT index (o + off) mod n
-//   foldn outputs o < foldn also add FFT index (o + N + off) mod n
-//         (reference-mode circular wrap when the FFT is padded, P != N)
 // Tile targets: complex line pairs per tile of the z passes (each pair is two real
 // lines), and kz columns per tile of the y and x passes.  Powers of two.
 #ifndef PVD_TILE_Z
@@ -26,6 +20,12 @@
 #define PVD_TILE_X 16
 #endif
 
+// One FFT axis of one volume.
+//   n     FFT length P (7-smooth)
+//   N     logical length (activity grid extent)
+//   off   output o reads FFT index (o + off) mod n
+//   foldn outputs o < foldn also add FFT index (o + N + off) mod n
+//         (reference-mode circular wrap when the FFT is padded, P != N)
 struct pvd_axis_d {
     int n;
     int N;

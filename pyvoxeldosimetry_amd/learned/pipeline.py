@@ -131,15 +131,19 @@ class LearnedCorrector:
         return self._bufs[key]
 
     def pack(self, act, rho, dp):
+        """BF16 network input; returns (x, max dp as a 1-element device tensor).
+        The scales are read on the device: no host synchronisation."""
         import torch
         x = self._buf(act.shape, "x", torch.bfloat16, 16)
-        sa = 1.0 / max(float(act.max().item()), 1e-30)
-        sd = 1.0 / max(float(dp.max().item()), 1e-30)
+        amax = act.max().reshape(1)
+        dmax = dp.max().reshape(1)
         nat.check(self.lib.pvd_cnn_pack_inputs(ctypes.c_void_p(act.data_ptr()), ctypes.c_void_p(rho.data_ptr()),
-                                               ctypes.c_void_p(dp.data_ptr()), ctypes.c_float(sa), ctypes.c_float(sd),
-                                               ctypes.c_void_p(x.data_ptr()), act.numel(), self._stream()),
+                                               ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(amax.data_ptr()),
+                                               ctypes.c_void_p(dmax.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                               act.numel(), self._stream()),
                   "pvd_cnn_pack_inputs")
-        return x, sd
+        self._keep = (amax, dmax)          # alive until the next call, past the asynchronous kernel
+        return x, dmax
 
     def train_step(self, act, rho, target, lr: float = 1e-3, mask_frac: float = 0.05):
         """One Adam step on one (activity, density, target dose) triple; returns
@@ -148,10 +152,10 @@ class LearnedCorrector:
         import torch
         shape = tuple(act.shape)
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
-        x, sd = self.pack(act, rho, dp)
+        x, dmax = self.pack(act, rho, dp)
         self.net.forward(x, shape)
         dpf = dp.reshape(-1)
-        mask = (dpf * sd >= mask_frac).float()
+        mask = (dpf >= mask_frac * dmax).float()
         ones = torch.ones_like(dpf)
         ratio = torch.where(mask > 0, target.reshape(-1) / torch.clamp(dpf, min=1e-30), ones)
         sums = self.net.loss_and_grad(ones, ratio, mask, shape)
@@ -160,30 +164,31 @@ class LearnedCorrector:
         return sums
 
     def correct(self, act, rho, r_override=None) -> Correction:
-        """CNN-corrected dose with fallback.  `r_override` (tests) replaces the
-        network output, e.g. to force an out-of-bound correction."""
+        """CNN-corrected dose with fallback (one host synchronisation, for the
+        fallback decision).  `r_override` (tests) replaces the network output,
+        e.g. to force an out-of-bound correction.  `plain` and `r` of the result
+        are views of buffers the next call on this shape overwrites."""
         import torch
         shape = tuple(act.shape)
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
         if r_override is None:
-            x, _ = self.pack(act, rho, dp)
+            x, dmax = self.pack(act, rho, dp)
             r = self.net.forward(x, shape, keep_activations=False)
         else:
+            dmax = dp.max().reshape(1)
             r = r_override.reshape(-1).contiguous()
         out = torch.empty(shape, dtype=torch.float32, device=self.device)
-        exceed = torch.zeros(1, dtype=torch.float32, device=self.device)
-        region_min = 0.1 * float(dp.max().item())
+        counts = torch.zeros(2, dtype=torch.int64, device=self.device)
         nat.check(self.lib.pvd_cnn_apply_correction(ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(r.data_ptr()),
-                                                    ctypes.c_float(self.bound), ctypes.c_float(region_min),
-                                                    ctypes.c_void_p(out.data_ptr()),
-                                                    ctypes.c_void_p(exceed.data_ptr()), dp.numel(), self._stream()),
+                                                    ctypes.c_float(self.bound), ctypes.c_void_p(dmax.data_ptr()),
+                                                    ctypes.c_float(0.1), ctypes.c_void_p(out.data_ptr()),
+                                                    ctypes.c_void_p(counts.data_ptr()), dp.numel(), self._stream()),
                   "pvd_cnn_apply_correction")
-        n_ex = int(exceed.item())
-        n_region = int((dp >= region_min).sum().item())
+        n_ex, n_region = (int(v) for v in counts.tolist())
         fallback = n_region > 0 and n_ex > self.max_exceed_frac * n_region
         if fallback:
             out.copy_(dp)
-        return Correction(out, dp.clone(), r.reshape(shape).clone(), n_ex, fallback)
+        return Correction(out, dp, r.reshape(shape), n_ex, fallback)
 
 
 def voxel_errors(dose, ref, region_frac: float = 0.1):

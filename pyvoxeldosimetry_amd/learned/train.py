@@ -138,6 +138,8 @@ def main(argv=None):
     torch.save(corr.net.state_dict(), args.save)
     if args.export:
         corr.net.save_weights(args.export, {"nuclide": "Y90", "voxel_mm": 2.0, "shape": args.shape,
+                                            "command": "python -m pyvoxeldosimetry_amd.learned.train "
+                                                       + " ".join(sys.argv[1:] if argv is None else argv),
                                             "train_seeds": f"{args.seed0}..{args.seed0 + args.n_train - 1}",
                                             "train_histories": args.train_histories, "epochs": args.epochs,
                                             "adam_steps": corr.net.step, "eval": json.dumps(ev)})

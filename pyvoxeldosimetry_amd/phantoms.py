@@ -197,3 +197,28 @@ def synthetic_kernel(voxel_mm: float, grid: Sequence[int], length_mm: float = 2.
     r = np.sqrt(((x - c[0]) * voxel_mm) ** 2 + ((y - c[1]) * voxel_mm) ** 2 + ((z - c[2]) * voxel_mm) ** 2)
     k = np.exp(-r / length_mm) / (r ** 2 + core_mm2)
     return k / k.sum()
+
+
+def ct_hu_phantom(seed: int = 6, shape=(512, 512, 400), metal: bool = True) -> np.ndarray:
+    """Synthetic torso CT in HU (float32): air -1000 outside the body, soft
+    tissue 40 +- 15, fat -100, lungs -850 +- 40, spine 700 +- 150 and, with
+    `metal`, two dense implants of 3000 HU (a hip-like sphere and a thin rod)."""
+    rng = np.random.default_rng(seed)
+    sx, sy, sz = shape
+    ct = np.full(shape, -1000.0, dtype=np.float32)
+    body = _ellipsoid(shape, (sx / 2, sy / 2, sz / 2), (0.45 * sx, 0.35 * sy, 0.6 * sz))
+    ct[body] = 40.0
+    fat = body & ~_ellipsoid(shape, (sx / 2, sy / 2, sz / 2), (0.41 * sx, 0.31 * sy, 0.58 * sz))
+    ct[fat] = -100.0
+    for cy in (0.32, 0.68):
+        ct[_ellipsoid(shape, (sx / 2, cy * sy, 0.7 * sz), (0.2 * sx, 0.12 * sy, 0.2 * sz))] = -850.0
+    x, y = np.ogrid[:sx, :sy]
+    spine = ((x - 0.75 * sx) ** 2 + (y - sy / 2) ** 2) <= (0.04 * sx) ** 2
+    ct[spine[:, :, None] & body] = 700.0
+    noise = rng.normal(0.0, 1.0, size=shape).astype(np.float32)
+    scale = np.where(ct == -850.0, 40.0, np.where(ct == 700.0, 150.0, np.where(ct == -1000.0, 5.0, 15.0)))
+    ct += (noise * scale).astype(np.float32)
+    if metal:
+        _fill_sphere(ct, (0.5 * sx, 0.3 * sy, 0.2 * sz), max(2, int(0.03 * sx)), 3000.0)
+        ct[int(0.6 * sx), int(0.6 * sy), int(0.1 * sz):int(0.5 * sz)] = 3000.0
+    return ct

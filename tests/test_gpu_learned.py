@@ -87,6 +87,30 @@ def test_conv_forward_all_epilogues(cin):
 
 
 @pytest.mark.parametrize("cin", [16, 32])
+def test_conv_forward_many_blocks_per_workgroup(cin):
+    """A grid with more output blocks than workgroups (ragged in x and y): every
+    workgroup walks several blocks, prefetching the next halo during the MFMA loop."""
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    shape = (35, 37, 64)
+    _say("cnn conv fwd persistent", cin, shape)
+    lib = nat.load()
+    rng = np.random.default_rng(7 + cin)
+    n = int(np.prod(shape))
+    x = ocnn.bf16_round(rng.normal(size=shape + (cin,)).astype(np.float32))
+    w = ocnn.bf16_round(rng.normal(scale=0.1, size=(27, 32, cin)).astype(np.float32))
+    b = rng.normal(scale=0.1, size=32).astype(np.float32)
+    xt, wt, bt = bf(x).reshape(n, cin), bf(w), torch.from_numpy(b).cuda()
+    r = torch.empty(n, dtype=torch.float32, device="cuda")
+    _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_LINEAR_F32, shape, y32=r)
+    z = ocnn.conv3(x, w) + b
+    np.testing.assert_allclose(r.cpu().numpy().reshape(shape), z[..., 0], rtol=1e-5, atol=1e-4)
+    y = torch.empty((n, 32), dtype=torch.bfloat16, device="cuda")
+    _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RELU, shape, y=y)
+    np.testing.assert_allclose(y.float().cpu().numpy().reshape(shape + (32,)), np.maximum(z, 0), rtol=8e-3, atol=1e-3)
+
+
+@pytest.mark.parametrize("cin", [16, 32])
 def test_weight_and_input_gradients(cin):
     import ctypes
     import torch

@@ -43,6 +43,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(nat.PlanParams) == 4 * 12
     assert ctypes.sizeof(nat.DpkTerm) == 24
     assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8      # 9 ints, padding, size_t
+    # pvd_tissue_params: 2 ints + 2x8 + 2x32 floats + int + float, then two doubles
+    assert ctypes.sizeof(nat.TissueParams) == 4 * (2 + 16 + 64 + 2) + 16
+    assert nat.TissueParams.sigma.offset == 4 * (2 + 16 + 64 + 2)
 
 
 def test_product_path_fails_loudly_without_gpu():
@@ -171,3 +174,33 @@ def test_shipped_cnn_weights_are_well_formed():
         w5 = w[off5:off5 + 27 * 32 * 32].reshape(27, 32, 32)
         assert not w5[:, 1:, :].any() and w5[:, 0, :].any()
         assert {"nuclide", "voxel_mm", "train_seeds", "command"} <= set(meta)
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof/offsetof of every ABI struct as gcc lays it out == the ctypes mirror."""
+    import shutil
+    import subprocess
+    from pyvoxeldosimetry_amd import _native as nat
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    fields = {"pvd_plan_params": (nat.PlanParams, ["flags"]), "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes"]),
+              "pvd_dpk_term": (nat.DpkTerm, ["param"]), "pvd_mc_params": (nat.McParams, ["active", "n_active"]),
+              "pvd_tissue_params": (nat.TissueParams, ["sigma", "truncate"]),
+              "pvd_tac_params": (nat.TacParams, ["tol", "fail_params", "t_int"])}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pvd.h"', '#include "pvd_learned.h"',
+             '#include "pvd_tissue.h"', '#include "pvd_tac.h"', "int main(void) {"]
+    for name, (_, fl) in fields.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for f in fl:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for name, (cls, fl) in fields.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for f in fl:
+            assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, (name, f)
