@@ -50,89 +50,117 @@ __device__ __forceinline__ double fill_source(float c, float metal) {
     return (double)c;
 }
 
-// Smoothed value at voxel idx, computed by the whole wavefront: lane l takes the
-// taps l, l+64, ... of the (2r+1)^3 window, then an FP64 butterfly reduction.
-// Called with every lane of the wavefront active and the same idx.
-__device__ double metal_fill_wave(const TissueArgs &a, long long idx, int lane) {
-    const int z = (int)(idx % a.n2);
-    const long long xy = idx / a.n2;
-    const int y = (int)(xy % a.n1);
-    const int x = (int)(xy / a.n1);
-    const int r = a.radius, side = 2 * r + 1, taps = side * side * side;
-    double part = 0.0;
-    for (int tap = lane; tap < taps; tap += 64) {
-        const int dz = tap % side - r, dy = (tap / side) % side - r, dx = tap / (side * side) - r;
-        const long long j = ((long long)reflect(x + dx, a.n0) * a.n1 + reflect(y + dy, a.n1)) * a.n2 +
-                            reflect(z + dz, a.n2);
-        part += a.w[dx + r] * a.w[dy + r] * a.w[dz + r] * fill_source(a.ct[j], a.metal);
-    }
+// Smoothed values of V consecutive voxels of one z-row (x, y, z0 .. z0+V-1),
+// for the voxels flagged in `which`.  For each of the (2r+1)^2 (dx, dy) rows the
+// lane issues all V+2r reflected z-loads of that row before using them, so a
+// metal voxel costs (2r+1)^2 load round trips, shared by the V voxels.  R is
+// the radius when known at compile time (the window then lives in registers),
+// or -1 for a run-time radius.
+template <int V, int R>
+__device__ void metal_fill_row(const TissueArgs &a, int x, int y, int z0, unsigned which, float *h) {
+    const int r = R >= 0 ? R : a.radius, side = 2 * r + 1;
+    constexpr int W = V + 2 * (R >= 0 ? R : PVD_TISSUE_MAX_RADIUS);
+    double acc[V];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-    return part;
-}
-
-__device__ __forceinline__ void classify(const TissueArgs &a, float v, unsigned char &bits, float &dens) {
-    unsigned b = 0;
-    for (int c = 0; c < a.ncls; ++c) b |= (unsigned)(v >= a.clo[c] && v <= a.chi[c]) << c;
-    bits = (unsigned char)b;
-    int m = 0;
-    for (int i = 1; i < a.nmat; ++i) m = (v >= a.mlo[i]) ? i : m;
-    dens = (v == v) ? a.mden[m] : v;
-}
-
-// Voxels are streamed 4 per lane; metal voxels found in a wavefront's chunk are
-// then filled one after another by the whole wavefront (wave-uniform loop).
-// The grid-stride loop advances whole wavefronts so every lane reaches the
-// shuffles; lanes past the end only skip their loads and stores.
-template <bool VEC>
-__global__ __launch_bounds__(256) void tissue_map(TissueArgs a, long long nitems) {
-    const int lane = threadIdx.x & 63;
-    const long long wave0 = (blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~63));
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    constexpr int V = VEC ? 4 : 1;
-    for (long long base = wave0; base < nitems; base += stride) {
-        const long long i = base + lane;
-        const bool valid = i < nitems;
-        float h[V];
-        if constexpr (VEC) {
-            const float4 q = valid ? reinterpret_cast<const float4 *>(a.ct)[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            h[0] = q.x;
-            h[1] = q.y;
-            h[2] = q.z;
-            h[3] = q.w;
-        } else {
-            h[0] = valid ? a.ct[i] : 0.f;
-        }
-        unsigned flags = 0;
+    for (int k = 0; k < V; ++k) acc[k] = 0.0;
+    for (int dx = -r; dx <= r; ++dx) {
+        const long long px = (long long)reflect(x + dx, a.n0) * a.n1;
+        for (int dy = -r; dy <= r; ++dy) {
+            const float *row = a.ct + (px + reflect(y + dy, a.n1)) * a.n2;
+            const double wxy = a.w[dx + r] * a.w[dy + r];
+            float win[W];
 #pragma unroll
-        for (int k = 0; k < V; ++k) flags |= (unsigned)(valid && a.handle && h[k] > a.metal) << k;
-        unsigned long long pending = __ballot(flags != 0);
-        while (pending) {
-            const int src = __ffsll((long long)pending) - 1;
-            const unsigned f = __shfl(flags, src, 64);
-            const long long i_src = base + src;
+            for (int j = 0; j < W; ++j)
+                if (R >= 0 || j < V + 2 * r) win[j] = row[reflect(z0 - r + j, a.n2)];
 #pragma unroll
             for (int k = 0; k < V; ++k) {
-                if (f & (1u << k)) {
-                    const double v = metal_fill_wave(a, i_src * V + k, lane);
-                    if (lane == src) h[k] = (float)v;
-                }
-            }
-            pending &= pending - 1;
-        }
-        if (!valid) continue;
-        unsigned char b[V];
-        float d[V];
+                if (!(which & (1u << k))) continue;
+                double s = 0.0;
+                if constexpr (R >= 0) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) classify(a, h[k], b[k], d[k]);
-        if constexpr (VEC) {
-            if (a.corrected) reinterpret_cast<float4 *>(a.corrected)[i] = make_float4(h[0], h[1], h[2], h[3]);
-            if (a.bits) reinterpret_cast<uchar4 *>(a.bits)[i] = make_uchar4(b[0], b[1], b[2], b[3]);
-            if (a.density) reinterpret_cast<float4 *>(a.density)[i] = make_float4(d[0], d[1], d[2], d[3]);
-        } else {
-            if (a.corrected) a.corrected[i] = h[0];
-            if (a.bits) a.bits[i] = b[0];
-            if (a.density) a.density[i] = d[0];
+                    for (int dz = 0; dz < 2 * R + 1; ++dz) s += a.w[dz] * fill_source(win[k + dz], a.metal);
+                } else {
+                    for (int dz = 0; dz < side; ++dz) s += a.w[dz] * fill_source(win[k + dz], a.metal);
+                }
+                acc[k] += wxy * s;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+        if (which & (1u << k)) h[k] = (float)acc[k];
+}
+
+// Class bits and density of one HU value.  The loops have compile-time trip
+// counts (NM materials, all PVD_TISSUE_MAX_CLASSES classes; the host pads unused
+// entries with bounds that never match) so the table stays in scalar registers.
+template <int NM>
+__device__ __forceinline__ void classify(const TissueArgs &a, float v, unsigned char &bits, float &dens) {
+    unsigned b = 0;
+#pragma unroll
+    for (int c = 0; c < PVD_TISSUE_MAX_CLASSES; ++c) b |= (unsigned)(v >= a.clo[c] && v <= a.chi[c]) << c;
+    bits = (unsigned char)b;
+    float d = a.mden[0];
+#pragma unroll
+    for (int i = 1; i < NM; ++i) d = (v >= a.mlo[i]) ? a.mden[i] : d;
+    dens = (v == v) ? d : v;
+}
+
+// Voxels are streamed V per lane (V = 4 with 16-byte aligned maps and n2 % 4 == 0,
+// so a lane's voxels share one z-row); a lane holding metal fills them in place.
+// Each lane issues the loads of kItems grid-stride items before using any, so a
+// wavefront keeps kItems x 64 x 4V bytes in flight.
+constexpr int kItems = 4;
+
+template <int V, int R, int NM>
+__device__ __forceinline__ void tissue_item(const TissueArgs &a, long long i, float *h) {
+    unsigned which = 0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) which |= (unsigned)(a.handle && h[k] > a.metal) << k;
+    if (which) {
+        const long long v0 = i * V;
+        const long long xy = v0 / a.n2;
+        metal_fill_row<V, R>(a, (int)(xy / a.n1), (int)(xy % a.n1), (int)(v0 % a.n2), which, h);
+    }
+    unsigned char b[V];
+    float d[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) classify<NM>(a, h[k], b[k], d[k]);
+    if constexpr (V == 4) {
+        if (a.corrected) reinterpret_cast<float4 *>(a.corrected)[i] = make_float4(h[0], h[1], h[2], h[3]);
+        if (a.bits) reinterpret_cast<uchar4 *>(a.bits)[i] = make_uchar4(b[0], b[1], b[2], b[3]);
+        if (a.density) reinterpret_cast<float4 *>(a.density)[i] = make_float4(d[0], d[1], d[2], d[3]);
+    } else {
+        if (a.corrected) a.corrected[i] = h[0];
+        if (a.bits) a.bits[i] = b[0];
+        if (a.density) a.density[i] = d[0];
+    }
+}
+
+template <bool VEC, int R, int NM>
+__global__ __launch_bounds__(256) void tissue_map(TissueArgs a, long long nitems) {
+    constexpr int V = VEC ? 4 : 1;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; i0 < nitems; i0 += kItems * stride) {
+        float h[kItems][V];
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const long long i = i0 + u * stride;
+            if (i >= nitems) continue;
+            if constexpr (VEC) {
+                const float4 q = reinterpret_cast<const float4 *>(a.ct)[i];
+                h[u][0] = q.x;
+                h[u][1] = q.y;
+                h[u][2] = q.z;
+                h[u][3] = q.w;
+            } else {
+                h[u][0] = a.ct[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const long long i = i0 + u * stride;
+            if (i < nitems) tissue_item<V, R, NM>(a, i, h[u]);
         }
     }
 }
@@ -163,13 +191,13 @@ extern "C" int pvd_tissue_map(const pvd_tissue_params *p, const float *ct, int n
     a.metal = p->metal_hu;
     a.ncls = p->n_classes;
     a.nmat = p->n_materials;
-    for (int c = 0; c < a.ncls; ++c) {
-        a.clo[c] = p->class_lo[c];
-        a.chi[c] = p->class_hi[c];
+    for (int c = 0; c < PVD_TISSUE_MAX_CLASSES; ++c) {          // unused classes never match
+        a.clo[c] = c < a.ncls ? p->class_lo[c] : INFINITY;
+        a.chi[c] = c < a.ncls ? p->class_hi[c] : -INFINITY;
     }
-    for (int i = 0; i < a.nmat; ++i) {
-        a.mlo[i] = p->mat_lo[i];
-        a.mden[i] = p->mat_density[i];
+    for (int i = 0; i < PVD_TISSUE_MAX_MATERIALS; ++i) {        // unused materials are never reached
+        a.mlo[i] = i < a.nmat ? p->mat_lo[i] : INFINITY;
+        a.mden[i] = p->mat_density[i < a.nmat ? i : a.nmat - 1];   // +inf HU keeps the last material
     }
     if (a.handle) {
         if (!(p->sigma > 0.0) || !(p->truncate >= 0.0))
@@ -184,12 +212,23 @@ extern "C" int pvd_tissue_map(const pvd_tissue_params *p, const float *ct, int n
     }
     const long long n = (long long)n0 * n1 * n2;
     hipStream_t st = (hipStream_t)stream;
-    const bool vec = (n % 4 == 0) && aligned(ct, 16) && (!corrected || aligned(corrected, 16)) &&
+    const bool vec = (n2 % 4 == 0) && aligned(ct, 16) && (!corrected || aligned(corrected, 16)) &&
                      (!density || aligned(density, 16)) && (!class_bits || aligned(class_bits, 4));
     const long long items = vec ? n / 4 : n;
-    const long long g = std::min<long long>((items + 255) / 256, 16384);
-    if (vec) hipLaunchKernelGGL(tissue_map<true>, dim3((unsigned)g), dim3(256), 0, st, a, items);
-    else hipLaunchKernelGGL(tissue_map<false>, dim3((unsigned)g), dim3(256), 0, st, a, items);
+    const long long g = std::min<long long>((items + 255) / 256, 4096);
+    const bool r4 = a.radius == 4;          // sigma 1, truncate 4: the reference's filter
+    const dim3 grid((unsigned)g), block(256);
+    if (a.nmat <= 16) {
+        if (vec && r4) hipLaunchKernelGGL((tissue_map<true, 4, 16>), grid, block, 0, st, a, items);
+        else if (vec) hipLaunchKernelGGL((tissue_map<true, -1, 16>), grid, block, 0, st, a, items);
+        else if (r4) hipLaunchKernelGGL((tissue_map<false, 4, 16>), grid, block, 0, st, a, items);
+        else hipLaunchKernelGGL((tissue_map<false, -1, 16>), grid, block, 0, st, a, items);
+    } else {
+        if (vec && r4) hipLaunchKernelGGL((tissue_map<true, 4, 32>), grid, block, 0, st, a, items);
+        else if (vec) hipLaunchKernelGGL((tissue_map<true, -1, 32>), grid, block, 0, st, a, items);
+        else if (r4) hipLaunchKernelGGL((tissue_map<false, 4, 32>), grid, block, 0, st, a, items);
+        else hipLaunchKernelGGL((tissue_map<false, -1, 32>), grid, block, 0, st, a, items);
+    }
     if (hipError_t e = hipGetLastError(); e != hipSuccess) return pvd_hip_error(e, "launch tissue map");
     return PVD_OK;
 }

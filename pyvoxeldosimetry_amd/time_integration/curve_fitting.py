@@ -28,9 +28,13 @@ NUKFIT_MODELS = {
 }
 LMFIT_MODELS = ("mono-exponential", "bi-exponential", "tri-exponential")
 PVD_TAC_MONO = 0
-BASIC_MAX_ITER = 200        # curve_fit's default maxfev = 200 (n + 1) evaluations for 2 parameters
-ADVANCED_MAX_ITER = 1000    # the reference passes maxfev=10000
-TOL = 1e-10
+# Iteration caps mirror curve_fit's evaluation budgets (one LM iteration ~ n + 1
+# evaluations with MINPACK's finite-difference Jacobian): default maxfev = 200 (n + 1)
+# for the basic fit, maxfev = 10000 for the advanced one.  The tolerance is
+# MINPACK's default ftol = xtol = 1.49012e-8 that curve_fit uses.
+BASIC_MAX_ITER = 200
+ADVANCED_MAXFEV = 10000
+TOL = 1.49012e-8
 
 
 def _lmfit_models(half_life):
@@ -147,7 +151,8 @@ class TimeCurveFitting:
         mid, p0 = models[model_name]
         lam = self.decay_constant if phys_half_life is None else np.log(2) / phys_half_life
         params, auc, status, shape = self._fit(times, activities, mid, p0, weight_factors, force_zero_at_t0,
-                                               ADVANCED_MAX_ITER, [0.0] * len(p0), 1, lambda_phys=float(lam))
+                                               ADVANCED_MAXFEV // (len(p0) + 1), [0.0] * len(p0), 1,
+                                               lambda_phys=float(lam))
         fit_params = params.cpu().numpy().astype(np.float64).reshape((len(p0), *shape))
         aucs = auc.cpu().numpy().astype(np.float64).reshape(shape)
         if return_fit_params:

@@ -1,9 +1,9 @@
 """GPU tests of the voxelwise TAC fits (include/pvd_tac.h) against oracle/tac.py
 (scipy.optimize.curve_fit per voxel, as the reference).
 
-Tolerances: both solvers converge to the weighted least-squares minimum; the
-GPU iterates to a relative tolerance of 1e-10 and stores float32, scipy stops at
-its default 1.5e-8, so well-conditioned parameters agree to ~1e-5 relative.
+Tolerances: both solvers converge to the weighted least-squares minimum with
+MINPACK's default relative tolerance 1.49e-8; the GPU stores float32, so
+well-conditioned parameters agree to ~1e-5 relative.
 For models whose minimum is not unique on the sampled times (f3, bi-exponential),
 the test checks that the GPU reaches a cost no higher than scipy's on at least
 95 % of the voxels (a different descent path may end in another local minimum)."""
