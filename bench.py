@@ -90,12 +90,13 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 256 MiB (reference for rocprof byte counters)")
-    ap.add_argument("--config", default="ct", choices=["ct", "batch", "torso", "tissue", "cnn", "tac"],
+    ap.add_argument("--config", default="ct", choices=["ct", "batch", "torso", "tissue", "cnn", "tac", "resample"],
                     help="ct: 256x256x200 single volume (default, the headline metric); batch: 48 uneven "
                          "phantoms x 4 timepoints in one graph; torso: 512x512x400, 61^3, 6 timepoints; "
                          "tissue: HU -> metal fill, class masks and density of a 512x512x400 CT; "
                          "cnn: CNN-corrected Y-90 dose of a 128^3 phantom (time, MFMA use, error vs Monte Carlo); "
-                         "tac: voxelwise mono-exponential TAC fit + integral of 256x256x200 voxels x 4 timepoints")
+                         "tac: voxelwise mono-exponential TAC fit + integral of 256x256x200 voxels x 4 timepoints; "
+                         "resample: trilinear 512x512x400 @0.98 mm -> 2 mm grid and back")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -128,6 +129,8 @@ def main():
         return bench_cnn(args)
     if args.config == "tac":
         return bench_tac(args)
+    if args.config == "resample":
+        return bench_resample(args)
 
     from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
     from pyvoxeldosimetry_amd.phantoms import ct_phantom
@@ -417,6 +420,63 @@ def bench_tac(args):
         res["speedup_vs_cpu_reference"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
         pg = params[:, :m].cpu().numpy().astype(np.float64)
         res["max_rel_param_err_vs_curve_fit"] = float(np.max(np.abs(pg - pe) / np.abs(pe)))
+    print(json.dumps(res), flush=True)
+
+
+def bench_resample(args):
+    """Trilinear resampling of a 512x512x400 map at 0.98 mm onto the 2 mm kernel grid
+    (251x251x196) and of that grid back to 0.98 mm: the voxel-size round trip of the dose path."""
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd.phantoms import torso_phantom
+    from pyvoxeldosimetry_amd.resample import output_shape_for, resample_device
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ph = torso_phantom()
+    a = torch.from_numpy(ph.activity[0]).to(dev)
+    fine, coarse = (0.98,) * 3, (2.0,) * 3
+    cshape = output_shape_for(a.shape, fine, coarse)
+    down = torch.empty(cshape, dtype=torch.float32, device=dev)
+    up = torch.empty_like(a)
+    step = lambda: (resample_device(a, fine, coarse, "linear", cshape, out=down),  # noqa: E731
+                    resample_device(down, coarse, fine, "linear", a.shape, out=up))
+    ms = _timed(step, args.steps, args.warmup)
+    n_out = down.numel() + up.numel()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(20):
+        resample_device(down, coarse, fine, "linear", a.shape, out=up)
+    ev1.record()
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / 20
+    alg = 4 * (down.numel() + up.numel())      # read the coarse grid once, write the fine one
+    res = {"metric": "Gvoxel/s trilinear resampling 512x512x400 @0.98 mm <-> 2 mm (output voxels)",
+           "value": round(n_out / (ms * 1e-3) / 1e9, 3), "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic (torso phantom seed 5, first timepoint)",
+           "config": {"workload": "resample down to %s and back up to %s" % (list(cshape), list(a.shape))},
+           "roofline": {"bound": "hbm", "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": None, "kernel": "resample_kernel (coarse -> fine)", "kernel_ms": round(k_ms, 5),
+                        "algorithmic_bytes": alg}}
+    if not args.no_cpu:
+        from oracle import resample as ors
+        sub = ph.activity[0][:, :, :100]
+        t0 = time.perf_counter()
+        ors.resample(sub, fine, coarse, "linear")
+        t_cpu = time.perf_counter() - t0
+        n_cpu = int(np.prod(output_shape_for(sub.shape, fine, coarse)))
+        res["cpu_baseline"] = {"value": n_cpu / t_cpu / 1e9, "unit": "Gvoxel/s", "cores": 1, "kind": "port",
+                               "sample": f"scipy map_coordinates order 1 (the oracle) on a 512x512x100 slab, "
+                                         f"{t_cpu:.2f} s"}
+        got = down.cpu().numpy()
+        exp = ors.resample(ph.activity[0][:64, :64, :64], fine, coarse, "linear")
+        gsub = resample_device(torch.from_numpy(np.ascontiguousarray(ph.activity[0][:64, :64, :64])).to(dev),
+                               fine, coarse, "linear").cpu().numpy()
+        res["max_abs_err_vs_oracle"] = float(np.abs(gsub - exp).max())
+        del got
     print(json.dumps(res), flush=True)
 
 

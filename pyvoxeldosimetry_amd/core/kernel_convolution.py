@@ -15,6 +15,10 @@ Config keys read here (all optional):
   ``fix_kernel_centre`` Y-90: 0 instead of the reference's NaN centre voxel
   ``kernel_scale``     multiplies every dose (e.g. the JSON scaling_factor 1e-12)
   ``out_dtype``        'float32' (default) or 'float64' for NumPy outputs
+  ``resample``         True (default): a voxel size other than kernel_resolution
+                       is handled as the reference intends (:63-68): activity (and
+                       density) are resampled onto the kernel grid, convolved, and
+                       the dose is resampled back (trilinear, GPU).  False: raise.
 """
 from __future__ import annotations
 
@@ -65,13 +69,17 @@ class KernelConvolutionCalculator(DosimetryCalculator):
             self._kernel_np = self.kernel_device.cpu().numpy()
         return self._kernel_np
 
-    def _check_voxel_size(self, voxel_size):
+    def _check_voxel_size(self, voxel_size) -> bool:
+        """True when voxel_size equals the kernel grid; raises if it differs and
+        resampling is switched off (the reference calls a stub returning None)."""
         if len(voxel_size) != 3:
             raise ValueError("voxel_size must be a tuple of length 3.")
-        if tuple(float(v) for v in voxel_size) != (float(self.kernel_resolution),) * 3:
-            # the reference calls a resampling stub here that returns None (kernel_convolution.py:63-68,108-115)
+        if tuple(float(v) for v in voxel_size) == (float(self.kernel_resolution),) * 3:
+            return True
+        if not self.config.get("resample", True):
             raise ValueError(f"voxel_size {tuple(voxel_size)} differs from kernel_resolution "
-                             f"{self.kernel_resolution}; activity resampling is not implemented")
+                             f"{self.kernel_resolution} and config['resample'] is False")
+        return False
 
     def plan_for(self, shape) -> ConvPlan:
         """Plan cached per grid shape (autotuned when ``config['autotune']``)."""
@@ -108,12 +116,23 @@ class KernelConvolutionCalculator(DosimetryCalculator):
         Extra (optional) inputs: ``density_map`` (g/cc, applies rho_w/rho) and
         ``scale`` (e.g. a decay factor), both fused into the last GPU pass.
         """
-        self._check_voxel_size(voxel_size)
+        same = self._check_voxel_size(voxel_size)
         a, kind = to_device_f32(activity_map, self.kernel_device.device)
-        plan = self.plan_for(a.shape)
         d = self._density(density_map, a.shape, a.device)
-        out = plan.dose(a, density=d, scale=self.kernel_scale * float(scale))
+        if not same:
+            out = self._dose_on_kernel_grid(a, d, voxel_size, self.kernel_scale * float(scale))
+        else:
+            out = self.plan_for(a.shape).dose(a, density=d, scale=self.kernel_scale * float(scale))
         return from_device(out, kind, self.out_dtype)
+
+    def _dose_on_kernel_grid(self, a, d, voxel_size, scale):
+        """Activity (and density) onto the kernel grid, convolve, dose back onto the input grid."""
+        from ..resample import resample_device
+        kres = (float(self.kernel_resolution),) * 3
+        ak = self._resample_activity(a, voxel_size, kres)
+        dk = None if d is None else resample_device(d, voxel_size, kres, "linear", output_shape=ak.shape)
+        dose_k = self.plan_for(ak.shape).dose(ak, density=dk, scale=scale)
+        return resample_device(dose_k, kres, voxel_size, "linear", output_shape=a.shape)
 
     def calculate_dose_rates(self, activity_maps: List, voxel_size, density_map=None, scales=None):
         """One dose-rate map per activity map (the reference's ``dose_rate_maps``)."""
@@ -127,16 +146,26 @@ class KernelConvolutionCalculator(DosimetryCalculator):
         By linearity this is one convolution of the trapezoid-weighted activity,
         with the weighted sum folded into the first FFT pass.
         """
-        self._check_voxel_size(voxel_size)
+        same = self._check_voxel_size(voxel_size)
         if len(activity_maps) != len(time_points):
             raise ValueError("Number of activity maps must match number of time points")
         acts, kinds = zip(*[to_device_f32(a, self.kernel_device.device) for a in activity_maps])
         shape = acts[0].shape
         if any(a.shape != shape for a in acts):
             raise ValueError("All activity maps must have the same dimensions")
-        plan = self.plan_for(shape)
         d = self._density(density_map, shape, acts[0].device)
         from .. import _native as nat
+        if not same:
+            from .activity_sampler import trapezoid_weights, weighted_sum_device
+            w = [x * 3600.0 for x in trapezoid_weights(time_points)]
+            acc = None
+            for i in range(0, len(acts), nat.PVD_MAX_TIMEPOINTS):
+                part = weighted_sum_device(list(acts[i:i + nat.PVD_MAX_TIMEPOINTS]),
+                                           w[i:i + nat.PVD_MAX_TIMEPOINTS])
+                acc = part if acc is None else acc.add_(part)
+            out = self._dose_on_kernel_grid(acc, d, voxel_size, self.kernel_scale)
+            return from_device(out, kinds[0], self.out_dtype)
+        plan = self.plan_for(shape)
         if len(acts) <= nat.PVD_MAX_TIMEPOINTS:
             out = plan.dose_timeseries(list(acts), list(time_points), density=d, scale=self.kernel_scale)
         else:
@@ -164,7 +193,15 @@ class KernelConvolutionCalculator(DosimetryCalculator):
                                         scale=_UNIT_SECONDS[time_units])
 
     def _resample_activity(self, activity_map, input_voxel_size, output_voxel_size):
-        raise NotImplementedError("activity resampling is not implemented")
+        """``kernel_convolution.py:108-115`` (a stub there): per-voxel activity onto a
+        grid of `output_voxel_size`, trilinear on the GPU, total activity kept by the
+        voxel-volume ratio.  Returns a GPU tensor."""
+        from ..resample import output_shape_for, resample_device
+        a, _ = to_device_f32(activity_map, self.kernel_device.device)
+        ratio = float(np.prod(output_voxel_size)) / float(np.prod(input_voxel_size))
+        return resample_device(a, input_voxel_size, output_voxel_size, "linear",
+                               output_shape=output_shape_for(a.shape, input_voxel_size, output_voxel_size),
+                               scale=ratio)
 
     def close(self):
         for p in self._plans.values():

@@ -45,8 +45,9 @@ def test_calculator_cropped_default_kernel_fixed_centre():
     assert got.dtype == np.float64
     exp = conv.ref_conv(act, dpk.y90_kernel(1.0, (64, 64, 64), fix_centre=True))
     assert rel(got, exp) <= MAX_REL
+    strict = KernelConvolutionCalculator("Y90", "water", 1.0, {"fix_kernel_centre": True, "resample": False})
     with pytest.raises(ValueError, match="kernel_resolution"):
-        calc.calculate_dose_rate(act, (2.0, 2.0, 2.0))
+        strict.calculate_dose_rate(act, (2.0, 2.0, 2.0))
 
 
 def test_calculator_absorbed_dose_time_series_lu177():
