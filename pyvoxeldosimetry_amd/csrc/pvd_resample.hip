@@ -30,10 +30,15 @@ __device__ __forceinline__ bool axis_pos(double c, int n, int &i0, float &f) {
 }
 
 // One workgroup per output row (i, j): the x/y neighbours and weights are
-// uniform across the row, the threads stride along z.
+// uniform across the row, the threads stride along z.  Workgroups are handed
+// to the 8 XCDs round-robin, so block b works on row chunk (b % 8) * G/8 + b / 8:
+// each XCD walks a contiguous band of rows and neighbouring output rows find
+// their shared input rows in that XCD's L2.
 __global__ __launch_bounds__(256) void resample_kernel(ResampleArgs a) {
     const int rows = a.m[0] * a.m[1];
-    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int G = gridDim.x;
+    const int b = (G % 8 == 0) ? (int)(blockIdx.x % 8) * (G / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    for (int row = b; row < rows; row += G) {
         const int i = row / a.m[1], j = row - i * a.m[1];
         float *orow = a.out + (long long)row * a.m[2];
         int b0, b1;
