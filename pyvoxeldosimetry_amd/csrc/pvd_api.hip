@@ -275,6 +275,9 @@ int fft_setup(pvd_plan *p) {
     pick_tile(Py, PVD_TILE_Y, 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
     pick_tile(Px, PVD_TILE_X, 0, &tkx, &bx, &ldsx);
+    // room for the padded exchange array of the register-resident 256-point path
+    if (Py == 256 && tky == 16) ldsy += PVD_SQ256_EXTRA_LDS;
+    if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
     // the y and x passes move 16-byte column pairs and read whole tiles of columns
     if (tky < 2 || tkx < 2) return fail(PVD_ENOTSUP, "FFT length too large for a two-column tile");
     // rows padded to 128 bytes and to whole column tiles

@@ -228,6 +228,134 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 }
 
 // ---------------------------------------------------------------------------------
+// Length-256 column FFTs held in registers (the y and x passes of 256-point
+// axes, 16 columns per tile, 256 threads).  Four-step split 256 = 16 x 16:
+// thread (l = column, g) owns the 16 elements g, g+16, ... of its column, does a
+// 16-point DFT in registers, applies W_256^(g k1), exchanges through LDS once
+// ([k1][n2][l], rows padded by kSqPad), and finishes with a second 16-point DFT:
+// X[k1 + 16 k2] = sum_n2 W_16^(n2 k2) W_256^(n2 k1) sum_n1 x[16 n1 + n2] W_16^(n1 k1).
+// One barrier per transform instead of one per radix stage, and no generic
+// index arithmetic.  The inverse of data held as X[k1 + 16 k2] (k2 in
+// registers) is the same split in the other order:
+// x[j1 + 16 j2] = sum_k1 W_16^(-j2 k1) W_256^(-j1 k1) sum_k2 X[k1 + 16 k2] W_16^(-j1 k2).
+constexpr int kSqPad = PVD_SQ256_PAD;      // float2 padding between k1 planes (bank spread)
+constexpr int kSqPlane = 256 + kSqPad;
+
+__device__ __forceinline__ bool sq256_ok(const pvd_axis_d &ax, int lnl) {
+    return ax.n == 256 && lnl == 4 && blockDim.x == 256 && ax.foldn == 0;
+}
+
+// v[k1] = sum_n1 v[n1] W16^(+-n1 k1), then times W256^(+-g k1)
+template <bool INV>
+__device__ __forceinline__ void sq_first(float2 *v, int g, const float2 *tw) {
+    Dft<16, INV>::run(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        float2 w = tw[g * k];
+        if (INV) w.y = -w.y;
+        v[k] = cmul(v[k], w);
+    }
+}
+
+// forward 256-point transform of registers x[n1] = column element 16 n1 + g;
+// returns X[k1 + 16 k2] in v[k2] for k1 = g (after one LDS exchange)
+template <bool INV>
+__device__ __forceinline__ void sq_transform(float2 *v, int l, int g, const float2 *tw, float2 *s) {
+    sq_first<INV>(v, g, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) s[k1 * kSqPlane + g * 16 + l] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = s[g * kSqPlane + n2 * 16 + l];
+    Dft<16, INV>::run(v);
+}
+
+template <bool INV>
+__device__ void pass_y_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ay = v.ax[1];
+    const int Ny = ay.N, H = v.H, Hp = v.Hp;
+    const int ntc = (H + 15) >> 4;
+    const int x = tile / ntc;
+    const int c0 = (tile - x * ntc) << 4;
+    const int nc = min(16, H - c0);
+    float2 *g = v.work + ((long long)x * 256) * Hp + c0;
+    const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    load_twiddles(tw, ay);
+    const int rows_in = INV ? 256 : Ny;
+    float2 a[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int r = 16 * n1 + grp;
+        a[n1] = (r < rows_in && l < nc) ? g[(long long)r * Hp + l] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();                                   // twiddles visible
+    sq_transform<INV>(a, l, grp, tw, s);
+    if (l >= nc) return;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        const int k = grp + 16 * k2;
+        if (!INV) {
+            g[(long long)k * Hp + l] = a[k2];
+        } else {
+            const int o = k >= ay.off ? k - ay.off : k + 256 - ay.off;     // output row reading index k
+            if (o < Ny) g[(long long)o * Hp + l] = a[k2];
+        }
+    }
+}
+
+__device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ax = v.ax[0];
+    const int Nx = ax.N, H = v.H, Hp = v.Hp;
+    const int ntc = (H + 15) >> 4;
+    const int y = tile / ntc;
+    const int c0 = (tile - y * ntc) << 4;
+    const int nc = min(16, H - c0);
+    const int Py = v.ax[1].n;
+    const long long plane = (long long)Py * Hp;
+    float2 *g = v.work + (long long)y * Hp + c0;
+    const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    load_twiddles(tw, ax);
+    float2 a[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) {
+        const int r = 16 * n1 + grp;
+        a[n1] = (r < Nx && l < nc) ? g[r * plane + l] : make_float2(0.f, 0.f);
+    }
+    // kernel spectrum of rows k = grp + 16 k2 (read before the barrier, in flight with the FFT)
+    float2 sp[16];
+    if (v.sym) {
+        const float *sr = reinterpret_cast<const float *>(v.spec);
+        const long long pyh = Py / 2 + 1;
+        const long long row0 = (long long)min(y, Py - y) * Hp + c0 + l;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            const int k = grp + 16 * k2;
+            sp[k2] = make_float2(l < nc ? sr[(long long)min(k, 256 - k) * pyh * Hp + row0] : 0.f, 0.f);
+        }
+    } else {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            const int k = grp + 16 * k2;
+            sp[k2] = l < nc ? v.spec[(long long)y * Hp + c0 + k * plane + l] : make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();                                   // twiddles visible
+    sq_transform<false>(a, l, grp, tw, s);             // a[k2] = X[grp + 16 k2]
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) a[k2] = v.sym ? cscale(a[k2], sp[k2].x) : cmul(a[k2], sp[k2]);
+    // inverse: thread-local 16-point DFT over k2 and twiddle W256^(-j1 k1), k1 = grp
+    __syncthreads();                                   // every thread done reading s
+    sq_transform<true>(a, l, grp, tw, s);              // a[j2] = x[grp + 16 j2]
+    if (l >= nc) return;
+#pragma unroll
+    for (int j2 = 0; j2 < 16; ++j2) {
+        const int j = grp + 16 * j2;
+        const int o = j >= ax.off ? j - ax.off : j + 256 - ax.off;
+        if (o < Nx) g[o * plane + l] = a[j2];
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // FY / IY: tile = (x, block of 2^ltky consecutive kz columns); lines run along y.
 template <bool INV>
 __device__ void pass_y(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
@@ -421,10 +549,16 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     float2 *tw = smem;
     float2 *s = smem + ((v.ax[axis].n + 7) & ~7);
     if (PASS == PVD_PASS_FZ) pass_fz(v, tile, tw, s);
-    else if (PASS == PVD_PASS_FY) pass_y<false>(v, tile, tw, s);
-    else if (PASS == PVD_PASS_X) pass_x(v, tile, tw, s);
-    else if (PASS == PVD_PASS_IY) pass_y<true>(v, tile, tw, s);
-    else pass_iz(v, tile, tw, s);
+    else if (PASS == PVD_PASS_FY) {
+        if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<false>(v, tile, tw, s);
+        else pass_y<false>(v, tile, tw, s);
+    } else if (PASS == PVD_PASS_X) {
+        if (sq256_ok(v.ax[0], v.ltkx)) pass_x_sq(v, tile, tw, s);
+        else pass_x(v, tile, tw, s);
+    } else if (PASS == PVD_PASS_IY) {
+        if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<true>(v, tile, tw, s);
+        else pass_y<true>(v, tile, tw, s);
+    } else pass_iz(v, tile, tw, s);
 }
 
 }  // namespace

@@ -20,6 +20,12 @@
 #define PVD_TILE_X 16
 #endif
 
+// The register-resident 256-point column FFT of the y and x passes exchanges
+// through a [16][16*16 + PVD_SQ256_PAD] float2 LDS array, PVD_SQ256_PAD x 16 float2
+// more than a plain 16-column tile of 256 rows.
+#define PVD_SQ256_PAD 16
+#define PVD_SQ256_EXTRA_LDS (16 * PVD_SQ256_PAD * 8)
+
 // One FFT axis of one volume.
 //   n     FFT length P (7-smooth)
 //   N     logical length (activity grid extent)
