@@ -82,6 +82,30 @@ def test_graph_replays_track_new_inputs():
     b.close()
 
 
+def test_batch_with_256_point_volume_matches_single_plans():
+    """A 256x256 volume next to smaller ones: the batch launch uses the register-resident
+    256-point path for that volume and the generic path for the others, bit-exact with
+    single-volume plans."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    from pyvoxeldosimetry_amd.batch import DoseBatch
+    _say("batch with a 256-point volume")
+    shapes = [(256, 256, 24), (40, 36, 32), (64, 64, 30)]
+    rng = np.random.default_rng(44)
+    kernels = [dpk.lu177_kernel(2.0, (9, 9, 9)), dpk.y90_kernel(4.0, (11, 11, 11), fix_centre=True),
+               dpk.lu177_kernel(2.0, (9, 9, 9))]
+    acts = [rng.random(s).astype(np.float32) for s in shapes]
+    b = DoseBatch(shapes, kernels, mode="reference")
+    for i in range(len(shapes)):
+        b.set_volume(i, [dev(acts[i])], [1.0])
+    outs = [o.cpu().numpy() for o in b.run(graph=True)]
+    for i, shp in enumerate(shapes):
+        plan = ConvPlan(shp, kernels[i].shape, mode="reference", algo="fft")
+        plan.set_kernel(kernels[i])
+        np.testing.assert_array_equal(outs[i], plan.dose(dev(acts[i])).cpu().numpy(), err_msg=f"volume {i}")
+        assert conv.error_metrics(outs[i], conv.ref_conv(acts[i].astype(np.float64), kernels[i]))["max_rel"] <= MAX_REL
+    b.close()
+
+
 def test_batch_errors():
     from pyvoxeldosimetry_amd.batch import DoseBatch
     _say("batch errors")

@@ -294,3 +294,26 @@ def test_torso_config_512x512x400_61cubed_6_timepoints():
     exp = conv.epilogue(conv.linear_same(acc, k.cpu().numpy()), ph.density)
     m = assert_parity(got, exp, what="torso/linear")
     _say(f"torso max_rel={m['max_rel']:.2e} rms_rel={m['rms_rel']:.2e} fft_n={plan.fft_n}")
+
+
+@pytest.mark.parametrize("mode,shape,kshape,sym", [
+    ("linear", (241, 240, 40), (31, 31, 31), True),       # P = 256 on x and y, off = 0, zero rows past N
+    ("reference", (256, 256, 24), (9, 7, 5), False),     # P = N = 256, complex spectrum, origin anchor
+    ("reference", (256, 100, 20), (11, 11, 11), True),    # 256 on x only (y takes the generic path)
+])
+def test_register_256_point_path(mode, shape, kshape, sym):
+    """The register-resident 256-point y/x transforms: linear padding, a non-symmetric
+    kernel (general complex spectrum) and a mix with the generic path, vs the oracle."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("256-point register path", mode, shape, kshape)
+    rng = np.random.default_rng(sum(shape))
+    a = rng.random(shape).astype(np.float32)
+    rho = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+    k = dpk.lu177_kernel(1.0, kshape) if sym else rng.random(kshape)
+    plan = ConvPlan(shape, kshape, mode=mode, algo="fft")
+    plan.set_kernel(k)
+    assert 256 in plan.fft_n[:2] and plan.symmetric_spectrum == sym
+    oracle_fn = conv.ref_conv if mode == "reference" else conv.linear_same
+    exp = conv.epilogue(oracle_fn(a.astype(np.float64), np.asarray(k, np.float64)), rho)
+    got = plan.dose(dev_tensor(a), density=dev_tensor(rho)).cpu().numpy()
+    assert_parity(got, exp, what=f"{mode} {shape} {kshape}")
