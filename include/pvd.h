@@ -57,6 +57,9 @@ typedef struct pvd_plan_params {
     int algo;       /* PVD_ALGO_*                                               */
     int fft_n[3];   /* FFT lengths per axis, 0 = plan's choice (7-smooth)       */
     int flags;      /* PVD_FLAG_*                                               */
+    int tile[3];    /* lines per FFT tile: x pass, y pass (kz columns), z pass  */
+                    /* (complex line pairs); power of two <= 64, 0 = 16         */
+    int max_radix;  /* largest power-of-two Stockham radix: 16 or 8, 0 = 16     */
 } pvd_plan_params;
 
 typedef struct pvd_plan_info {
@@ -66,6 +69,8 @@ typedef struct pvd_plan_info {
     int kernel_used[3];     /* kernel extent after the reference's crop          */
     int n_passes;           /* kernel launches per dose call                     */
     size_t workspace_bytes; /* device bytes owned by the plan                    */
+    int tile[3];            /* lines per tile used by the x, y and z passes      */
+    int max_radix;          /* largest power-of-two radix used                   */
 } pvd_plan_info;
 
 /* Dose-point-kernel terms (the analytic generators of

@@ -38,12 +38,14 @@ ALGOS = {"auto": PVD_ALGO_AUTO, "fft": PVD_ALGO_FFT, "direct": PVD_ALGO_DIRECT}
 
 class PlanParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 3), ("k", ctypes.c_int * 3), ("mode", ctypes.c_int),
-                ("algo", ctypes.c_int), ("fft_n", ctypes.c_int * 3), ("flags", ctypes.c_int)]
+                ("algo", ctypes.c_int), ("fft_n", ctypes.c_int * 3), ("flags", ctypes.c_int),
+                ("tile", ctypes.c_int * 3), ("max_radix", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("symmetric_spectrum", ctypes.c_int), ("fft_n", ctypes.c_int * 3),
-                ("kernel_used", ctypes.c_int * 3), ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t)]
+                ("kernel_used", ctypes.c_int * 3), ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
+                ("tile", ctypes.c_int * 3), ("max_radix", ctypes.c_int)]
 
 
 class DpkTerm(ctypes.Structure):

@@ -1,9 +1,12 @@
-"""Measured choice of algorithm and FFT lengths per (grid, kernel, mode).
+"""Measured choice of algorithm, FFT lengths, tile shapes and radix per (grid, kernel, mode).
 
 Candidates: the FFT pipeline at the plan's default lengths and at a few other
 7-smooth lengths that do not alias, and the direct LDS convolution when the
-kernel extent allows.  Each is timed with HIP events on synthetic input; the
-winner is cached in a JSON file keyed by shape, kernel, mode and device name.
+kernel extent allows.  For the fastest FFT lengths the tuner then walks the
+tile size of each pass (8 / 16 / 32 lines per workgroup, z then y then x) and
+the largest Stockham radix (16 or 8), keeping a change only when it is at
+least 2 % faster.  Each candidate is timed with HIP events on synthetic input;
+the winner is cached in a JSON file keyed by shape, kernel, mode and device.
 """
 from __future__ import annotations
 
@@ -79,7 +82,7 @@ def time_plan(plan: ConvPlan, reps: int = 10, warmup: int = 3) -> float:
     return s.elapsed_time(e) / reps
 
 
-def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6) -> ConvPlan:
+def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, search_tiles: bool = True) -> ConvPlan:
     import torch
     dev = kernel_dev.device
     key = json.dumps([list(shape), list(kernel_shape), mode, torch.cuda.get_device_name(dev)])
@@ -87,8 +90,10 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6) -
         cache = _load_cache()
         hit = cache.get(key)
     if hit is not None:
-        plan = ConvPlan(shape, kernel_shape, mode=mode, algo=hit["algo"],
-                        fft_n=hit.get("fft_n") if hit["algo"] == "fft" else None, device=dev)
+        fft = hit["algo"] == "fft"
+        plan = ConvPlan(shape, kernel_shape, mode=mode, algo=hit["algo"], fft_n=hit.get("fft_n") if fft else None,
+                        tile=hit.get("tile") if fft else None, max_radix=hit.get("max_radix", 0) if fft else 0,
+                        device=dev)
         plan.set_kernel(kernel_dev)
         return plan
     cands = []
@@ -118,8 +123,39 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6) -
             plan.close()
     if best is None:
         raise RuntimeError(f"no usable convolution plan for shape {shape}, kernel {kernel_shape}")
+    tile, radix = [16, 16, 16], 16
+    if best[2] == "fft" and search_tiles:
+        fn = best[1].fft_n
+
+        def trial(t, r):
+            try:
+                p = ConvPlan(shape, kernel_shape, mode=mode, algo="fft", fft_n=fn, tile=t, max_radix=r, device=dev)
+            except (ValueError, NotImplementedError, MemoryError):
+                return None
+            p.set_kernel(kernel_dev)
+            return time_plan(p), p
+
+        for axis in (2, 1, 0):
+            for t in (8, 32):
+                cand = list(tile)
+                cand[axis] = t
+                res = trial(cand, radix)
+                if res is not None and res[0] < 0.98 * best[0]:
+                    best[1].close()
+                    best = (res[0], res[1], "fft", fn)
+                    tile = cand
+                elif res is not None:
+                    res[1].close()
+        res = trial(tile, 8)
+        if res is not None and res[0] < 0.98 * best[0]:
+            best[1].close()
+            best = (res[0], res[1], "fft", fn)
+            radix = 8
+        elif res is not None:
+            res[1].close()
     with _lock:
         cache = _load_cache()
-        cache[key] = {"algo": best[2], "fft_n": list(best[3]) if best[3] else None, "ms": best[0]}
+        cache[key] = {"algo": best[2], "fft_n": list(best[3]) if best[3] else None, "ms": best[0],
+                      "tile": tile if best[2] == "fft" else None, "max_radix": radix if best[2] == "fft" else 0}
         _save_cache(cache)
     return best[1]

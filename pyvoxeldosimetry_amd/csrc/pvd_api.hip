@@ -92,14 +92,12 @@ int next_smooth(int n) {
     return n;
 }
 
-int factor_radices(int n, unsigned char *radix) {
+// Stockham radices of n (largest power-of-two radix max_radix2 = 16 or 8)
+int factor_radices(int n, unsigned char *radix, int max_radix2) {
     int ns = 0;
     int p2 = 0;
     while (n % 2 == 0) { n /= 2; ++p2; }
-#ifndef PVD_MAX_RADIX2
-#define PVD_MAX_RADIX2 16   // largest power-of-two radix used by the Stockham plans
-#endif
-    while (PVD_MAX_RADIX2 >= 16 && p2 >= 4) { radix[ns++] = 16; p2 -= 4; }
+    while (max_radix2 >= 16 && p2 >= 4) { radix[ns++] = 16; p2 -= 4; }
     while (p2 >= 3) { radix[ns++] = 8; p2 -= 3; }
     if (p2 == 3) radix[ns++] = 8;
     else if (p2 == 2) radix[ns++] = 4;
@@ -172,6 +170,8 @@ struct pvd_plan {
     int N[3]{}, K[3]{}, Ku[3]{};   // grid, kernel, kernel extent used (crop)
     int P[3]{};
     int lo[3]{}, s[3]{};           // kernel placement (first used sample, shift)
+    int tile[3]{};                 // tile targets of the x, y, z passes
+    int max_radix = 16;            // largest power-of-two Stockham radix
     bool has_kernel = false;
     pvd_vol_d vol{};               // descriptor template
     launch_cfg cfg[PVD_NPASS_FFT];
@@ -234,7 +234,7 @@ int axis_setup(pvd_plan *p) {
         pvd_axis_d &ax = p->vol.ax[a];
         ax.n = P;
         ax.N = N;
-        ax.nst = factor_radices(P, ax.radix);
+        ax.nst = factor_radices(P, ax.radix, p->max_radix);
         if (ax.nst < 0 || ax.nst > PVD_MAXSTAGE) return fail(PVD_EINVAL, "cannot factor FFT length %d", P);
         // output o reads FFT index (o - s) mod P in reference mode, o in linear mode
         ax.off = ref ? ((P - (p->s[a] % P)) % P) : 0;
@@ -270,11 +270,11 @@ int fft_setup(pvd_plan *p) {
     const int Px = p->P[0], Py = p->P[1], Pz = p->P[2];
     v.H = Pz / 2 + 1;
     int lz, bz; size_t ldsz;
-    pick_tile(Pz, PVD_TILE_Z, 1, &lz, &bz, &ldsz);
+    pick_tile(Pz, p->tile[2], 1, &lz, &bz, &ldsz);
     int tky, by; size_t ldsy;
-    pick_tile(Py, PVD_TILE_Y, 0, &tky, &by, &ldsy);
+    pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
-    pick_tile(Px, PVD_TILE_X, 0, &tkx, &bx, &ldsx);
+    pick_tile(Px, p->tile[0], 0, &tkx, &bx, &ldsx);
     // room for the padded exchange array of the register-resident 256-point path
     if (Py == 256 && tky == 16) ldsy += PVD_SQ256_EXTRA_LDS;
     if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
@@ -424,9 +424,18 @@ int pvd_plan_create(pvd_plan **out, const pvd_plan_params *q) {
     }
     if (q->mode != PVD_MODE_REFERENCE && q->mode != PVD_MODE_LINEAR) return fail(PVD_EINVAL, "bad mode %d", q->mode);
     if (q->algo < PVD_ALGO_AUTO || q->algo > PVD_ALGO_DIRECT) return fail(PVD_EINVAL, "bad algo %d", q->algo);
+    for (int a = 0; a < 3; ++a) {
+        const int t = q->tile[a];
+        if (t < 0 || t > PVD_TILE_MAX || (t & (t - 1)))
+            return fail(PVD_EINVAL, "tile[%d]=%d must be 0 or a power of two <= %d", a, t, PVD_TILE_MAX);
+    }
+    if (q->max_radix != 0 && q->max_radix != 8 && q->max_radix != 16)
+        return fail(PVD_EINVAL, "max_radix must be 0, 8 or 16 (got %d)", q->max_radix);
     pvd_plan *p = new (std::nothrow) pvd_plan();
     if (!p) return fail(PVD_ENOMEM, "host allocation");
     p->prm = *q;
+    for (int a = 0; a < 3; ++a) p->tile[a] = q->tile[a] ? q->tile[a] : PVD_TILE_DEFAULT;
+    p->max_radix = q->max_radix ? q->max_radix : 16;
     int rc = PVD_OK;
     do {
         if (hipGetDevice(&p->dev) != hipSuccess) { rc = fail(PVD_EHIP, "no HIP device"); break; }
@@ -476,6 +485,11 @@ int pvd_plan_info_get(const pvd_plan *p, pvd_plan_info *info) {
     }
     info->n_passes = p->algo == PVD_ALGO_FFT ? PVD_NPASS_FFT : 1;
     info->workspace_bytes = p->bytes;
+    const bool fft = p->algo == PVD_ALGO_FFT;
+    info->tile[0] = fft ? 1 << p->vol.ltkx : 0;
+    info->tile[1] = fft ? 1 << p->vol.ltky : 0;
+    info->tile[2] = fft ? 1 << p->vol.llz : 0;
+    info->max_radix = fft ? p->max_radix : 0;
     return PVD_OK;
 }
 

@@ -92,13 +92,13 @@ __device__ __forceinline__ void store_col_tile(float2 *__restrict__ g, long long
 }
 
 // ---------------------------------------------------------------------------------
-// Offsets in the workspace of the (up to 32) real z-lines of a z-pass tile.
+// Offsets in the workspace of the (up to 2 * PVD_TILE_MAX) real z-lines of a z-pass tile.
 __device__ __forceinline__ void tile_line_offsets(long long *lineoff, long long L0, int nreal, int Ny, int Py,
                                                   int Hp) {
-    if (threadIdx.x < nreal) {
-        const int L = (int)(L0 + threadIdx.x);
+    for (int i = threadIdx.x; i < nreal; i += blockDim.x) {
+        const int L = (int)(L0 + i);
         const int x = fast_div(L, Ny, 1.0f / (float)Ny);
-        lineoff[threadIdx.x] = ((long long)x * Py + (L - x * Ny)) * Hp;
+        lineoff[i] = ((long long)x * Py + (L - x * Ny)) * Hp;
     }
 }
 
@@ -120,7 +120,7 @@ struct ZVec {
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
 // holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1).
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[2 * PVD_TILE_Z];
+    __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
@@ -446,7 +446,7 @@ __device__ __forceinline__ float iz_value(const float2 *s, const pvd_axis_d &az,
 }
 
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[2 * PVD_TILE_Z];
+    __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
     const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
     const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;

@@ -8,17 +8,11 @@
 
 #define PVD_MAXSTAGE 16
 
-// Tile targets: complex line pairs per tile of the z passes (each pair is two real
-// lines), and kz columns per tile of the y and x passes.  Powers of two.
-#ifndef PVD_TILE_Z
-#define PVD_TILE_Z 16
-#endif
-#ifndef PVD_TILE_Y
-#define PVD_TILE_Y 16
-#endif
-#ifndef PVD_TILE_X
-#define PVD_TILE_X 16
-#endif
+// Tiles: complex line pairs per tile of the z passes (each pair is two real
+// lines), kz columns per tile of the y and x passes.  Powers of two, at most
+// PVD_TILE_MAX (plan parameter; 16 by default).
+#define PVD_TILE_DEFAULT 16
+#define PVD_TILE_MAX 64
 
 // The register-resident 256-point column FFT of the y and x passes exchanges
 // through a [16][16*16 + PVD_SQ256_PAD] float2 LDS array, PVD_SQ256_PAD x 16 float2

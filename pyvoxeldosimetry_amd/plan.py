@@ -32,7 +32,10 @@ class ConvPlan:
 
     def __init__(self, shape: Sequence[int], kernel_shape: Sequence[int], mode: str = "reference",
                  algo: str = "auto", fft_n: Optional[Sequence[int]] = None, profile: bool = False,
-                 device=None, general_spectrum: bool = False):
+                 device=None, general_spectrum: bool = False, tile: Optional[Sequence[int]] = None,
+                 max_radix: int = 0):
+        """``tile``: lines per FFT tile of the (x, y, z) passes (powers of two, 0 = 16);
+        ``max_radix``: largest power-of-two Stockham radix (16 or 8, 0 = 16)."""
         import torch
         nat.require_device()
         self.lib = nat.load()
@@ -54,6 +57,8 @@ class ConvPlan:
         prm.fft_n[:] = tuple(int(x) for x in fft_n) if fft_n else (0, 0, 0)
         prm.flags = (nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum
                                                                  else 0)
+        prm.tile[:] = tuple(int(t) for t in tile) if tile else (0, 0, 0)
+        prm.max_radix = int(max_radix)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             nat.check(self.lib.pvd_plan_create(ctypes.byref(h), ctypes.byref(prm)), "pvd_plan_create")
@@ -65,6 +70,8 @@ class ConvPlan:
         self.kernel_used = tuple(info.kernel_used)
         self.n_passes = info.n_passes
         self.workspace_bytes = int(info.workspace_bytes)
+        self.tile = tuple(info.tile)
+        self.max_radix = int(info.max_radix)
         self.symmetric_spectrum = False
         self._kernel_dev = None
 
@@ -72,7 +79,7 @@ class ConvPlan:
     def info(self) -> dict:
         return {"algo": self.algo, "mode": self.mode, "fft_n": self.fft_n, "kernel_used": self.kernel_used,
                 "n_passes": self.n_passes, "workspace_bytes": self.workspace_bytes,
-                "symmetric_spectrum": self.symmetric_spectrum}
+                "symmetric_spectrum": self.symmetric_spectrum, "tile": self.tile, "max_radix": self.max_radix}
 
     def set_kernel(self, kernel) -> None:
         """FP64 dose kernel (torch tensor on the device or anything numpy accepts)."""

@@ -133,4 +133,11 @@ def test_autotuned_plan_matches(tmp_path, monkeypatch):
     exp = conv.ref_conv(a, dpk.lu177_kernel(2.0, (9, 9, 9)))
     assert rel(got, exp) <= MAX_REL
     assert (tmp_path / "tune.json").exists()
+    import json
+    entry = next(iter(json.load(open(tmp_path / "tune.json")).values()))
+    if entry["algo"] == "fft":
+        assert len(entry["tile"]) == 3 and entry["max_radix"] in (8, 16)
+    # a second calculator takes the cached choice and gives the same bits
+    calc2 = KernelConvolutionCalculator("Lu177", "water", 2.0, {"kernel_grid": (9, 9, 9), "autotune": True})
+    np.testing.assert_array_equal(calc2.calculate_dose_rate(a, (2.0, 2.0, 2.0)), got)
     del kc

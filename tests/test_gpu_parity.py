@@ -317,3 +317,21 @@ def test_register_256_point_path(mode, shape, kshape, sym):
     exp = conv.epilogue(oracle_fn(a.astype(np.float64), np.asarray(k, np.float64)), rho)
     got = plan.dose(dev_tensor(a), density=dev_tensor(rho)).cpu().numpy()
     assert_parity(got, exp, what=f"{mode} {shape} {kshape}")
+
+
+@pytest.mark.parametrize("tile,max_radix", [((8, 8, 8), 16), ((32, 32, 32), 8), ((16, 32, 8), 8)])
+def test_tile_and_radix_choices_keep_parity(tile, max_radix):
+    """Every tile shape / radix the autotuner may pick gives the oracle's result
+    (256-point axes fall back to the generic path when their tile is not 16)."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("tiles/radix", tile, max_radix)
+    rng = np.random.default_rng(5)
+    k = dpk.lu177_kernel(2.0, (9, 9, 9))
+    for shape in ((256, 64, 40), (45, 52, 33)):
+        a = rng.random(shape).astype(np.float32)
+        plan = ConvPlan(shape, (9, 9, 9), mode="reference", algo="fft", tile=tile, max_radix=max_radix)
+        plan.set_kernel(k)
+        info = plan.info()
+        assert info["max_radix"] == max_radix and all(t <= w for t, w in zip(info["tile"], tile))
+        assert_parity(plan.dose(dev_tensor(a)).cpu().numpy(), conv.ref_conv(a.astype(np.float64), k),
+                      what=f"tiles {tile} radix {max_radix} {shape}")

@@ -40,9 +40,9 @@ def test_binding_loads_and_reports_version_without_gpu():
 
 def test_struct_layouts_match_header():
     from pyvoxeldosimetry_amd import _native as nat
-    assert ctypes.sizeof(nat.PlanParams) == 4 * 12
+    assert ctypes.sizeof(nat.PlanParams) == 4 * 16
     assert ctypes.sizeof(nat.DpkTerm) == 24
-    assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8      # 9 ints, padding, size_t
+    assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8 + 4 * 4    # 9 ints, padding, size_t, 4 ints
     # pvd_tissue_params: 2 ints + 2x8 + 2x32 floats + int + float, then two doubles
     assert ctypes.sizeof(nat.TissueParams) == 4 * (2 + 16 + 64 + 2) + 16
     assert nat.TissueParams.sigma.offset == 4 * (2 + 16 + 64 + 2)
@@ -183,7 +183,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     from pyvoxeldosimetry_amd import _native as nat
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
-    fields = {"pvd_plan_params": (nat.PlanParams, ["flags"]), "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes"]),
+    fields = {"pvd_plan_params": (nat.PlanParams, ["flags", "tile", "max_radix"]),
+              "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes", "tile", "max_radix"]),
               "pvd_dpk_term": (nat.DpkTerm, ["param"]), "pvd_mc_params": (nat.McParams, ["active", "n_active"]),
               "pvd_tissue_params": (nat.TissueParams, ["sigma", "truncate"]),
               "pvd_tac_params": (nat.TacParams, ["tol", "fail_params", "t_int"])}
