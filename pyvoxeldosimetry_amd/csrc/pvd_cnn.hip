@@ -112,16 +112,24 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
         if (b + (int)gridDim.x < total) fetch(b + gridDim.x);
         const int ox = x0 + wx, oy = y0 + wy;
         if (ox >= a.n0 || oy >= a.n1) continue;      // wave-uniform
-        f32x16 acc = {};
+        // two independent accumulation chains (by tap parity), so consecutive MFMAs
+        // of a wave do not wait for each other's results
+        f32x16 acc0 = {}, acc1 = {};
         for (int tap = 0; tap < 27; ++tap) {
             const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
             const char *hrow = hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * VS + kh * 16;
             const char *wrow = wl + (tap * 32 + m) * WS + kh * 16;
 #pragma unroll
-            for (int cc = 0; cc < CIN / 16; ++cc)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32), acc,
-                                                              0, 0, 0);
+            for (int cc = 0; cc < CIN / 16; ++cc) {
+                if (((tap + cc) & 1) == 0)
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32),
+                                                                   acc0, 0, 0, 0);
+                else
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32),
+                                                                   acc1, 0, 0, 0);
+            }
         }
+        const f32x16 acc = acc0 + acc1;
         const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
         const int co = m;
         const float bco = a.bias ? a.bias[co] : 0.0f;
