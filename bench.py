@@ -51,10 +51,10 @@ def pass_bytes(plan, T: int, density: bool):
     }
 
 
-def read_traffic(kernel_name: str):
-    """HBM bytes per launch of `kernel_name` from the committed rocprof summary, if present."""
+def read_traffic(kernel_name: str, summary: str = TRAFFIC_FILE):
+    """HBM bytes per launch of `kernel_name` from a committed rocprof summary, if present."""
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(os.path.join(ROOT, "profiles", summary) if not os.path.isabs(summary) else summary).read()
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch\[" + re.escape(kernel_name) + r"\]\s*=\s*([0-9.eE+]+)", text)
@@ -355,7 +355,8 @@ def bench_cnn(args):
                         "unit": "TFLOP/s", "frac": round(alg / (ms_net * 1e-3) / 1e12 / peak, 4),
                         "executed_TFLOPs": round(executed / (ms_net * 1e-3) / 1e12, 1),
                         "executed_frac": round(executed / (ms_net * 1e-3) / 1e12 / peak, 4),
-                        "traffic": None, "kernel": "cnn_conv_fwd (6 launches)", "kernel_ms": round(ms_net, 5),
+                        "traffic": read_traffic("cnn_forward_6_launches", "cnn_forward_128cubed.md"),
+                        "kernel": "cnn_conv_fwd (6 launches)", "kernel_ms": round(ms_net, 5),
                         "algorithmic_flops": alg},
            "error_vs_monte_carlo_64cubed": ev}
     if not args.no_cpu:
@@ -459,7 +460,8 @@ def bench_resample(args):
            "config": {"workload": "resample down to %s and back up to %s" % (list(cshape), list(a.shape))},
            "roofline": {"bound": "hbm", "achieved": round(alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": None, "kernel": "resample_kernel (coarse -> fine)", "kernel_ms": round(k_ms, 5),
+                        "traffic": read_traffic("resample_kernel_coarse_to_fine", "resample_512x512x400.md"),
+                        "kernel": "resample_kernel (coarse -> fine)", "kernel_ms": round(k_ms, 5),
                         "algorithmic_bytes": alg}}
     if not args.no_cpu:
         from oracle import resample as ors
@@ -512,7 +514,8 @@ def bench_tissue(args):
            "config": {"workload": "tissue map 512x512x400 f32 HU, metal fill sigma 1 (r=4), 5 classes, "
                                   "16-material density", "metal_voxels": int((ct_np > 2000).sum())},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "tissue_map4",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": read_traffic("tissue_map4", "tissue_map_512x512x400.md"), "kernel": "tissue_map4",
                         "kernel_ms": round(k_ms, 5), "algorithmic_bytes": alg}}
     if not args.no_cpu:
         from oracle import tissue as ot
