@@ -43,6 +43,7 @@ extern "C" {
 
 #define PVD_FLAG_PROFILE 1  /* record HIP events around every pass (pvd_plan_timings) */
 #define PVD_FLAG_GENERAL_SPECTRUM 2  /* keep the full complex spectrum even for symmetric kernels */
+#define PVD_FLAG_STOCKHAM 4  /* LDS Stockham stages on every axis (no register-resident transforms) */
 
 #define PVD_MAX_TIMEPOINTS 16
 #define PVD_MAX_FFT_LEN    2048

@@ -25,6 +25,7 @@ PVD_ALGO_FFT = 1
 PVD_ALGO_DIRECT = 2
 PVD_FLAG_PROFILE = 1
 PVD_FLAG_GENERAL_SPECTRUM = 2
+PVD_FLAG_STOCKHAM = 4
 PVD_MAX_TIMEPOINTS = 16
 PVD_MAX_FFT_LEN = 2048
 

@@ -3,9 +3,10 @@
 Candidates: the FFT pipeline at the plan's default lengths and at a few other
 7-smooth lengths that do not alias, and the direct LDS convolution when the
 kernel extent allows.  For the fastest FFT lengths the tuner then walks the
-tile size of each pass (8 / 16 / 32 lines per workgroup, z then y then x) and
-the largest Stockham radix (16 or 8), keeping a change only when it is at
-least 2 % faster.  Each candidate is timed with HIP events on synthetic input;
+tile size of each pass (8 / 16 / 32 lines per workgroup, z then y then x), the
+largest Stockham radix (16 or 8) and the LDS Stockham stages in place of the
+register-resident transforms, keeping a change only when it is at least 2 %
+faster.  Each candidate is timed with HIP events on synthetic input;
 the winner is cached in a JSON file keyed by shape, kernel, mode and device.
 """
 from __future__ import annotations
@@ -93,7 +94,7 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
         fft = hit["algo"] == "fft"
         plan = ConvPlan(shape, kernel_shape, mode=mode, algo=hit["algo"], fft_n=hit.get("fft_n") if fft else None,
                         tile=hit.get("tile") if fft else None, max_radix=hit.get("max_radix", 0) if fft else 0,
-                        device=dev)
+                        stockham=bool(hit.get("stockham", False)) and fft, device=dev)
         plan.set_kernel(kernel_dev)
         return plan
     cands = []
@@ -123,13 +124,14 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
             plan.close()
     if best is None:
         raise RuntimeError(f"no usable convolution plan for shape {shape}, kernel {kernel_shape}")
-    tile, radix = [16, 16, 16], 16
+    tile, radix, stockham = [16, 16, 16], 16, False
     if best[2] == "fft" and search_tiles:
         fn = best[1].fft_n
 
-        def trial(t, r):
+        def trial(t, r, lds_only=False):
             try:
-                p = ConvPlan(shape, kernel_shape, mode=mode, algo="fft", fft_n=fn, tile=t, max_radix=r, device=dev)
+                p = ConvPlan(shape, kernel_shape, mode=mode, algo="fft", fft_n=fn, tile=t, max_radix=r,
+                             stockham=lds_only, device=dev)
             except (ValueError, NotImplementedError, MemoryError):
                 return None
             p.set_kernel(kernel_dev)
@@ -153,9 +155,17 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
             radix = 8
         elif res is not None:
             res[1].close()
+        res = trial(tile, radix, True)
+        if res is not None and res[0] < 0.98 * best[0]:
+            best[1].close()
+            best = (res[0], res[1], "fft", fn)
+            stockham = True
+        elif res is not None:
+            res[1].close()
     with _lock:
         cache = _load_cache()
         cache[key] = {"algo": best[2], "fft_n": list(best[3]) if best[3] else None, "ms": best[0],
-                      "tile": tile if best[2] == "fft" else None, "max_radix": radix if best[2] == "fft" else 0}
+                      "tile": tile if best[2] == "fft" else None, "max_radix": radix if best[2] == "fft" else 0,
+                      "stockham": stockham}
         _save_cache(cache)
     return best[1]

@@ -280,6 +280,25 @@ int fft_setup(pvd_plan *p) {
     if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
     // the y and x passes move 16-byte column pairs and read whole tiles of columns
     if (tky < 2 || tkx < 2) return fail(PVD_ENOTSUP, "FFT length too large for a two-column tile");
+    // register-resident transforms: the 256-point y/x path (checked again in the
+    // kernel) and the z two-step P = A * B for 16-line tiles whose block covers
+    // 16*A threads in step 2 and 16*B items in two rounds in step 1
+    const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
+    v.ax[0].reg = regs ? 1 : 0;
+    v.ax[1].reg = regs ? 1 : 0;
+    pvd_axis_d &az = v.ax[2];
+    az.reg = 0;
+    az.ra = az.ra_log = az.rb = 0;
+    for (int c = 1; regs && lz == 16 && c < PVD_ZREG_CODES; ++c) {
+        const int A = kZregA[c], B = kZregB[c];
+        if (A * B == Pz && 16 * A <= bz && 16 * B <= 2 * bz) {
+            az.reg = c;
+            az.ra = A;
+            az.ra_log = ilog2(A);
+            az.rb = B;
+            break;
+        }
+    }
     // rows padded to 128 bytes and to whole column tiles
     v.Hp = round_up(v.H, std::max(16, std::max(tky, tkx)));
     v.llz = ilog2(lz);

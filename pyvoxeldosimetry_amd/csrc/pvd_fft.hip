@@ -102,6 +102,78 @@ __device__ __forceinline__ void tile_line_offsets(long long *lineoff, long long 
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Register two-step transform of the z passes, n = A * B, for 16 lines in LDS
+// (element (line q, index k) at s[k*17 + q]).  Natural order in; the result
+// X[k1 + A k2] is left in row k1*B + k2 (zrow() maps an index to its row):
+//   step 1: thread (q, b) reads rows b + B n1 (n1 < A), DFT-A in registers,
+//           multiplies by W_n^(b k1) (b k1 < n, table in LDS) and writes the
+//           same rows back, Y[k1][b] in row k1*B + b;
+//   step 2: thread (q, k1) reads rows k1*B .. k1*B + B-1, DFT-B in registers
+//           (compile-time twiddles) and writes X[k1 + A k2] to row k1*B + k2.
+// Every thread writes only the rows it read, so one barrier after each step.
+// Needs 16*A <= blockDim and 16*B <= 2*blockDim; entered after a barrier.
+template <int A, int B, bool INV>
+__device__ __forceinline__ void zreg_fft(float2 *__restrict__ s, const float2 *__restrict__ tw) {
+    constexpr int LD = 17;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int w = threadIdx.x + it * blockDim.x;
+        const int b = w >> 4, q = w & 15;
+        if (b < B) {
+            float2 *p = s + b * LD + q;
+            float2 v[A];
+#pragma unroll
+            for (int n1 = 0; n1 < A; ++n1) v[n1] = p[n1 * B * LD];
+            dft_any<A, INV>(v);
+            p[0] = v[0];
+#pragma unroll
+            for (int k1 = 1; k1 < A; ++k1) {
+                float2 t = tw[b * k1];
+                if (INV) t.y = -t.y;
+                p[k1 * B * LD] = (b == 0) ? v[k1] : cmul(v[k1], t);
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const int w = threadIdx.x;
+        const int k1 = w >> 4, q = w & 15;
+        if (k1 < A) {
+            float2 *p = s + k1 * B * LD + q;
+            float2 v[B];
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) v[n2] = p[n2 * LD];
+            dft_any<B, INV>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) p[k2 * LD] = v[k2];
+        }
+    }
+    __syncthreads();
+}
+
+// Row of LDS that holds transform index k after z_fft.
+__device__ __forceinline__ int zrow(const pvd_axis_d &az, int k) {
+    return az.reg ? (k & (az.ra - 1)) * az.rb + (k >> az.ra_log) : k;
+}
+
+// z-axis FFT over the lines of a z-pass tile: the register two-step where the
+// plan selected one (16 lines per tile), else the LDS Stockham stages.
+template <bool INV>
+__device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis_d &az, const float2 *tw) {
+    switch (az.reg) {
+        case 1: zreg_fft<8, 16, INV>(s, tw); break;
+        case 2: zreg_fft<8, 20, INV>(s, tw); break;
+        case 3: zreg_fft<8, 24, INV>(s, tw); break;
+        case 4: zreg_fft<8, 25, INV>(s, tw); break;
+        case 5: zreg_fft<16, 16, INV>(s, tw); break;
+        case 6: zreg_fft<8, 30, INV>(s, tw); break;
+        case 7: zreg_fft<16, 20, INV>(s, tw); break;
+        case 8: zreg_fft<16, 25, INV>(s, tw); break;
+        default: fft_lines<INV>(s, lnl, ld, az, tw); break;
+    }
+}
+
 // float4 lane mapping shared by the z passes: 16 consecutive lanes cover 16
 // consecutive float4 of one line, neighbouring 16-lane groups take the next
 // lines, so a wavefront's LDS scatter/gather touches 4 line columns at once.
@@ -212,15 +284,15 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         }
     }
     __syncthreads();
-    fft_lines<false>(s, lnl, ld, az, tw);
+    z_fft<false>(s, lnl, ld, az, tw);
     // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i)
     const float inv_h = 1.0f / (float)H;
     for (int e = threadIdx.x; e < nreal * H; e += blockDim.x) {
         const int ln = fast_div(e, H, inv_h);
         const int k = e - ln * H;
         const int q = ln >> 1;
-        const float2 zk = s[k * ld + q];
-        const float2 zm = s[(k == 0 ? 0 : Pz - k) * ld + q];
+        const float2 zk = s[zrow(az, k) * ld + q];
+        const float2 zm = s[zrow(az, k == 0 ? 0 : Pz - k) * ld + q];
         const float2 r = (ln & 1) ? make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x))
                                   : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
         v.work[lineoff[ln] + k] = r;
@@ -242,7 +314,7 @@ constexpr int kSqPad = PVD_SQ256_PAD;      // float2 padding between k1 planes (
 constexpr int kSqPlane = 256 + kSqPad;
 
 __device__ __forceinline__ bool sq256_ok(const pvd_axis_d &ax, int lnl) {
-    return ax.n == 256 && lnl == 4 && blockDim.x == 256 && ax.foldn == 0;
+    return ax.reg && ax.n == 256 && lnl == 4 && blockDim.x == 256 && ax.foldn == 0;
 }
 
 // v[k1] = sum_n1 v[n1] W16^(+-n1 k1), then times W256^(+-g k1)
@@ -440,8 +512,9 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
 __device__ __forceinline__ float iz_value(const float2 *s, const pvd_axis_d &az, int ld, int o, int ln) {
     const int q = ln >> 1;
-    float val = reinterpret_cast<const float *>(&s[wrap_add(o, az.off, az.n) * ld + q])[ln & 1];
-    if (o < az.foldn) val += reinterpret_cast<const float *>(&s[((o + az.N + az.off) % az.n) * ld + q])[ln & 1];
+    float val = reinterpret_cast<const float *>(&s[zrow(az, wrap_add(o, az.off, az.n)) * ld + q])[ln & 1];
+    if (o < az.foldn)
+        val += reinterpret_cast<const float *>(&s[zrow(az, (o + az.N + az.off) % az.n) * ld + q])[ln & 1];
     return val;
 }
 
@@ -494,7 +567,7 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         }
     }
     __syncthreads();
-    fft_lines<true>(s, lnl, ld, az, tw);
+    z_fft<true>(s, lnl, ld, az, tw);
     const long long obase = L0 * Nz;
     const bool has_dens = v.dens != nullptr;
     const float sc = v.scale;

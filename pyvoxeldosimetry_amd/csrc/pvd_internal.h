@@ -26,13 +26,17 @@
 //   off   output o reads FFT index (o + off) mod n
 //   foldn outputs o < foldn also add FFT index (o + N + off) mod n
 //         (reference-mode circular wrap when the FFT is padded, P != N)
+//   reg   register-resident transform: on y/x, 1 = the 256-point 16x16 path may
+//         run; on z, index into kZregA/kZregB (0 = LDS Stockham stages)
+//   ra, ra_log, rb  the z split n = ra * rb (ra a power of two), when reg != 0
 struct pvd_axis_d {
     int n;
     int N;
     int nst;
     int off;
     int foldn;
-    int pad_;
+    int reg;
+    int ra, ra_log, rb, pad_;
     const float2 *tw;                 // n forward twiddles W_n^m, FP64-built, FP32-stored
     unsigned char radix[PVD_MAXSTAGE];
 };
@@ -211,6 +215,83 @@ template <bool INV> struct Dft<7, INV> {
         for (int k = 0; k < 7; ++k) v[k] = X[k];
     }
 };
+
+// ---------------------------------------------------------------------------------
+// Compile-time twiddles: cos / sin(2 pi j / N) from Taylor series in double on
+// [-pi, pi] (terms to x^41, error < 1e-16), rounded once to float.
+constexpr double kCtPi = 3.14159265358979323846;
+constexpr double ct_sin(double x) {
+    double term = x, sum = x;
+    for (int i = 1; i < 21; ++i) { term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0)); sum += term; }
+    return sum;
+}
+constexpr double ct_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i < 21; ++i) { term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i)); sum += term; }
+    return sum;
+}
+struct CtTwiddles { float c[64], s[64]; };
+template <int N> constexpr CtTwiddles ct_twiddles() {
+    CtTwiddles t{};
+    for (int j = 0; j < N; ++j) {
+        double a = 2.0 * kCtPi * j / N;
+        if (a > kCtPi) a -= 2.0 * kCtPi;
+        t.c[j] = (float)ct_cos(a);
+        t.s[j] = (float)ct_sin(a);
+    }
+    return t;
+}
+
+template <int N> constexpr bool dft_basic() {
+    return N == 2 || N == 3 || N == 4 || N == 5 || N == 7 || N == 8 || N == 16;
+}
+template <int N> constexpr int dft_first_factor() {
+    return (N % 8 == 0) ? 8 : (N % 4 == 0) ? 4 : (N % 5 == 0) ? 5 : (N % 3 == 0) ? 3 : (N % 7 == 0) ? 7 : 2;
+}
+
+// In-register DFT of any length N whose factors are 2, 3, 5, 7 (natural order in
+// and out): the basic lengths directly, others as one mixed-radix split
+// N = R1 * R2, n = R2 n1 + n2, k = k1 + R1 k2:
+// X[k] = sum_n2 W_R2^(n2 k2) W_N^(n2 k1) sum_n1 x[R2 n1 + n2] W_R1^(n1 k1),
+// with the W_N twiddles folded to constants at compile time.
+template <int N, bool INV>
+__device__ __forceinline__ void dft_any(float2 *v) {
+    if constexpr (dft_basic<N>()) {
+        Dft<N, INV>::run(v);
+    } else {
+        constexpr int R1 = dft_first_factor<N>(), R2 = N / R1;
+        static_assert(R1 * R2 == N && R2 > 1 && N <= 64, "dft_any: unsupported length");
+        constexpr CtTwiddles w = ct_twiddles<N>();
+        float2 u[N];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            float2 t[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) t[n1] = v[R2 * n1 + n2];
+            dft_any<R1, INV>(t);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) {
+                const int j = (n2 * k1) % N;
+                u[k1 * R2 + n2] = (j == 0) ? t[k1] : cmul(t[k1], make_float2(w.c[j], INV ? w.s[j] : -w.s[j]));
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            float2 t[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) t[n2] = u[k1 * R2 + n2];
+            dft_any<R2, INV>(t);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
+        }
+    }
+}
+
+// z lengths with a register two-step transform n = A * B (A a power of two):
+// code c of pvd_axis_d.reg selects (kZregA[c], kZregB[c]); 0 = none.
+#define PVD_ZREG_CODES 9
+constexpr int kZregA[PVD_ZREG_CODES] = {0, 8, 8, 8, 8, 16, 8, 16, 16};
+constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25};
 
 // ---------------------------------------------------------------------------------
 // a / d for 0 <= a < 2^22 and 1 <= d <= 2^16 from a float reciprocal plus one

@@ -33,9 +33,11 @@ class ConvPlan:
     def __init__(self, shape: Sequence[int], kernel_shape: Sequence[int], mode: str = "reference",
                  algo: str = "auto", fft_n: Optional[Sequence[int]] = None, profile: bool = False,
                  device=None, general_spectrum: bool = False, tile: Optional[Sequence[int]] = None,
-                 max_radix: int = 0):
+                 max_radix: int = 0, stockham: bool = False):
         """``tile``: lines per FFT tile of the (x, y, z) passes (powers of two, 0 = 16);
-        ``max_radix``: largest power-of-two Stockham radix (16 or 8, 0 = 16)."""
+        ``max_radix``: largest power-of-two Stockham radix (16 or 8, 0 = 16);
+        ``stockham``: LDS Stockham stages on every axis instead of the register-resident
+        transforms (256-point y/x, two-step z)."""
         import torch
         nat.require_device()
         self.lib = nat.load()
@@ -55,8 +57,9 @@ class ConvPlan:
         prm.mode = nat.MODES[mode]
         prm.algo = nat.ALGOS[algo]
         prm.fft_n[:] = tuple(int(x) for x in fft_n) if fft_n else (0, 0, 0)
-        prm.flags = (nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum
-                                                                 else 0)
+        prm.flags = ((nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum
+                                                                  else 0) | (nat.PVD_FLAG_STOCKHAM if stockham else 0))
+        self.stockham = bool(stockham)
         prm.tile[:] = tuple(int(t) for t in tile) if tile else (0, 0, 0)
         prm.max_radix = int(max_radix)
         h = ctypes.c_void_p()

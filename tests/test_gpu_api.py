@@ -136,7 +136,7 @@ def test_autotuned_plan_matches(tmp_path, monkeypatch):
     import json
     entry = next(iter(json.load(open(tmp_path / "tune.json")).values()))
     if entry["algo"] == "fft":
-        assert len(entry["tile"]) == 3 and entry["max_radix"] in (8, 16)
+        assert len(entry["tile"]) == 3 and entry["max_radix"] in (8, 16) and entry["stockham"] in (True, False)
     # a second calculator takes the cached choice and gives the same bits
     calc2 = KernelConvolutionCalculator("Lu177", "water", 2.0, {"kernel_grid": (9, 9, 9), "autotune": True})
     np.testing.assert_array_equal(calc2.calculate_dose_rate(a, (2.0, 2.0, 2.0)), got)

@@ -335,3 +335,60 @@ def test_tile_and_radix_choices_keep_parity(tile, max_radix):
         assert info["max_radix"] == max_radix and all(t <= w for t, w in zip(info["tile"], tile))
         assert_parity(plan.dose(dev_tensor(a)).cpu().numpy(), conv.ref_conv(a.astype(np.float64), k),
                       what=f"tiles {tile} radix {max_radix} {shape}")
+
+
+# z lengths with a register two-step transform P = A * B (A = 8 or 16), each in
+# reference mode (P = Nz) and one in linear mode (P chosen by the plan)
+ZREG_LENGTHS = [128, 160, 192, 200, 240, 256, 320, 400]
+
+
+@pytest.mark.parametrize("nz", ZREG_LENGTHS)
+def test_register_z_two_step_against_oracle_and_stockham(nz):
+    """Every z length with a register two-step transform matches the oracle, with a
+    symmetric (real-octant) and a general kernel, and gives the LDS Stockham
+    stages' result within FP32 rounding."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("register z two-step", nz)
+    rng = np.random.default_rng(nz)
+    shape = (6, 5, nz)
+    a = rng.random(shape).astype(np.float32)
+    rho = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+    for kshape, sym in (((5, 5, 7), True), ((3, 4, 9), False)):
+        k = dpk.lu177_kernel(1.0, kshape) if sym else rng.random(kshape)
+        exp = conv.epilogue(conv.ref_conv(a.astype(np.float64), np.asarray(k, np.float64)), rho)
+        outs = []
+        for lds_only in (False, True):
+            plan = ConvPlan(shape, kshape, mode="reference", algo="fft", stockham=lds_only)
+            plan.set_kernel(k)
+            assert plan.fft_n[2] == nz
+            got = plan.dose(dev_tensor(a), density=dev_tensor(rho)).cpu().numpy()
+            plan.close()
+            assert_parity(got, exp, what=f"nz={nz} k={kshape} stockham={lds_only}")
+            outs.append(got)
+        assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(exp).max()
+
+
+def test_register_z_two_step_linear_and_batch():
+    """Linear mode picks padded z lengths of the register table (160, 200; 210 is not), and a
+    batch of volumes with different z lengths runs each with its own transform."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    from pyvoxeldosimetry_amd.batch import DoseBatch
+    rng = np.random.default_rng(77)
+    k = dpk.lu177_kernel(2.0, (15, 15, 15))
+    shapes = [(20, 18, 146), (16, 24, 190), (9, 11, 200)]      # linear "same": P >= Nz + 7
+    acts = [rng.random(s).astype(np.float32) for s in shapes]
+    for s, a in zip(shapes, acts):
+        _say("register z linear", s)
+        plan = ConvPlan(s, k.shape, mode="linear", algo="fft")
+        plan.set_kernel(k)
+        assert plan.fft_n[2] in (160, 200, 210), plan.fft_n
+        assert_parity(plan.dose(dev_tensor(a)).cpu().numpy(), conv.linear_same(a.astype(np.float64), k),
+                      what=f"linear {s} fft_n={plan.fft_n}")
+        plan.close()
+    _say("register z batch")
+    b = DoseBatch(shapes, [k] * len(shapes), mode="linear")
+    for i, a in enumerate(acts):
+        b.set_volume(i, [dev_tensor(a)], [1.0])
+    outs = b.run(graph=True)
+    for s, a, o in zip(shapes, acts, outs):
+        assert_parity(o.cpu().numpy(), conv.linear_same(a.astype(np.float64), k), what=f"batch {s}")
