@@ -375,14 +375,23 @@ __device__ void pass_y_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     }
 }
 
+// X-pass tiles run through the y rows in mirror order 0, 1, Py-1, 2, Py-2, ...:
+// rows y and Py - y read the same rows of the real-octant spectrum, and
+// neighbouring column tiles share its 128-byte lines, so the tiles that share
+// spectrum lines are adjacent in the tile order.
+__device__ __forceinline__ int x_tile_row(int u, int Py) {
+    return u == 0 ? 0 : (u & 1) ? (u + 1) >> 1 : Py - (u >> 1);
+}
+
 __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &ax = v.ax[0];
     const int Nx = ax.N, H = v.H, Hp = v.Hp;
     const int ntc = (H + 15) >> 4;
-    const int y = tile / ntc;
-    const int c0 = (tile - y * ntc) << 4;
-    const int nc = min(16, H - c0);
     const int Py = v.ax[1].n;
+    const int u = tile / ntc;
+    const int y = x_tile_row(u, Py);
+    const int c0 = (tile - u * ntc) << 4;
+    const int nc = min(16, H - c0);
     const long long plane = (long long)Py * Hp;
     float2 *g = v.work + (long long)y * Hp + c0;
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
@@ -455,8 +464,9 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int Nx = ax.N, Px = ax.n, H = v.H, Hp = v.Hp;
     const int lnl = v.ltkx, TK = 1 << lnl;
     const int ntc = (H + TK - 1) >> lnl;
-    const int y = tile / ntc;
-    const int c0 = (tile - y * ntc) << lnl;
+    const int u = tile / ntc;
+    const int y = x_tile_row(u, v.ax[1].n);
+    const int c0 = (tile - u * ntc) << lnl;
     const int nc = min(TK, H - c0);
     const long long plane = (long long)v.ax[1].n * Hp;
     const long long off0 = (long long)y * Hp + c0;
@@ -639,7 +649,15 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
 // single volume: descriptor by value (kernarg segment, scalar loads)
 template <int PASS>
 __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
-    run_pass<PASS>(v, blockIdx.x);
+    int tile = blockIdx.x;
+    if (PASS == PVD_PASS_X) {
+        // Workgroups go to the 8 XCDs round-robin and the X grid is a multiple of 8:
+        // block b takes tile (b % 8) * (grid / 8) + b / 8, so each XCD walks a
+        // contiguous range of tiles and the tiles sharing spectrum lines meet in its L2.
+        tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+        if (tile >= v.tiles[PVD_PASS_X]) return;
+    }
+    run_pass<PASS>(v, tile);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
@@ -662,7 +680,9 @@ hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block
     switch (pass) {
         case PVD_PASS_FZ: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FZ>, dim3(grid), dim3(block), lds, st, v); break;
         case PVD_PASS_FY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, v); break;
-        case PVD_PASS_X: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_X>, dim3(grid), dim3(block), lds, st, v); break;
+        case PVD_PASS_X:   // grid rounded up to a multiple of 8 (XCD bands; surplus blocks exit)
+            hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_X>, dim3((grid + 7) & ~7), dim3(block), lds, st, v);
+            break;
         case PVD_PASS_IY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, v); break;
         default: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IZ>, dim3(grid), dim3(block), lds, st, v); break;
     }
