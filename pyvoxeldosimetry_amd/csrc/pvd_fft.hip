@@ -174,23 +174,21 @@ __device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis
     }
 }
 
-// float4 lane mapping shared by the z passes: 16 consecutive lanes cover 16
-// consecutive float4 of one line, neighbouring 16-lane groups take the next
-// lines, so a wavefront's LDS scatter/gather touches 4 line columns at once.
-struct ZVec {
-    int items, nreal16, nf4;
-    __device__ ZVec(int nreal, int Nz) : nreal16(nreal * 16), nf4(Nz >> 2) { items = ((nf4 + 15) >> 4) * nreal16; }
-    __device__ __forceinline__ bool decode(int i, float inv, int &ln, int &f) const {
-        const int chunk = fast_div(i, nreal16, inv);
-        const int rem = i - chunk * nreal16;
-        ln = rem >> 4;
-        f = (chunk << 4) + (rem & 15);
-        return i < items && f < nf4;
-    }
-};
+// Row step of z_fft's result per +16 in the index, and the drop in rows where an
+// index wraps past n: 16 / A and B for the register two-step (row of k is
+// (k mod A) * B + k / A), 16 and n for the Stockham stages (row of k is k).
+__device__ __forceinline__ int zrow_step(const pvd_axis_d &az) { return az.reg ? 16 >> az.ra_log : 16; }
+__device__ __forceinline__ int zrow_wrap(const pvd_axis_d &az) { return az.reg ? az.rb : az.n; }
+
+// The z passes move the elements t + 16 j (j = 0, 1, ...) of one line pair per
+// thread (q, t), 16 threads per pair: global accesses are 16 consecutive words
+// per line, LDS rows advance by a constant stride (zrow_step), both real lines
+// of a pair travel in one float2, and no index is divided.
+constexpr int kZBatch = 8;      // elements per line in flight per thread
 
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
-// holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1).
+// holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1),
+// real line 2q in the real and 2q+1 in the imaginary part of complex line q.
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
@@ -200,102 +198,110 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const long long nlines = (long long)v.ax[0].N * Ny;
     const long long L0 = (long long)tile * 2 * nl;
     const int nreal = (int)min((long long)(2 * nl), nlines - L0);
+    const int npair = (nreal + 1) >> 1;
     load_twiddles(tw, az);
     tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
-    // zero the padding: rows z >= Nz of every line, and the unused line pairs
+    // zero the padding rows z >= Nz of every line
     for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
-    const int npair = (nreal + 1) >> 1;
-    if (npair < nl || (nreal & 1)) {
-        for (int e = threadIdx.x; e < Nz * ld; e += blockDim.x) {
-            const int q = e % ld;
-            if (q >= nreal / 2) s[e] = make_float2(0.f, 0.f);
-        }
-        __syncthreads();
-    }
-    // weighted sum of the T maps over the tile's nreal*Nz contiguous floats
-    const long long base = L0 * Nz;
-    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps
-        const ZVec zv(nreal, Nz);
-        const float inv = 1.0f / (float)zv.nreal16;
-        for (int i0 = threadIdx.x; i0 < zv.items; i0 += 8 * blockDim.x) {
-            float4 a[8];
-            int lnv[8], fv[8];
+    // weighted sum of the T maps into rows z < Nz; pairs past the last line get zeros
+    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + 16 j)
+        const int nz4 = Nz >> 2;
+        constexpr int kB4 = kZBatch / 2;
+        for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
+            const int q = w >> 4, t = w & 15;
+            const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
+            const long long o0 = (L0 + 2 * q) * Nz;
+            for (int f0 = t; f0 < nz4; f0 += 16 * kB4) {
+                float4 a[kB4], b[kB4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                int ln, f;
-                const bool ok = zv.decode(i0 + u * blockDim.x, inv, ln, f);
-                lnv[u] = ok ? ln : -1;
-                fv[u] = f;
-                a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (ok) {
-                    const float4 x = reinterpret_cast<const float4 *>(v.act[0] + base + (long long)ln * Nz)[f];
-                    a[u] = make_float4(v.w[0] * x.x, v.w[0] * x.y, v.w[0] * x.z, v.w[0] * x.w);
+                for (int u = 0; u < kB4; ++u) {
+                    const int f = f0 + 16 * u;
+                    const float4 *m = reinterpret_cast<const float4 *>(v.act[0] + o0);
+                    const float4 x = (f < nz4 && h0) ? m[f] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float4 y = (f < nz4 && h1) ? m[nz4 + f] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float w0 = v.w[0];
+                    a[u] = make_float4(w0 * x.x, w0 * x.y, w0 * x.z, w0 * x.w);
+                    b[u] = make_float4(w0 * y.x, w0 * y.y, w0 * y.z, w0 * y.w);
                 }
-            }
-            for (int t = 1; t < v.T; ++t) {
+                for (int tt = 1; tt < v.T; ++tt) {
+                    const float4 *m = reinterpret_cast<const float4 *>(v.act[tt] + o0);
+                    const float wt = v.w[tt];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (lnv[u] >= 0) {
-                        const float4 x =
-                            reinterpret_cast<const float4 *>(v.act[t] + base + (long long)lnv[u] * Nz)[fv[u]];
-                        a[u].x = fmaf(v.w[t], x.x, a[u].x);
-                        a[u].y = fmaf(v.w[t], x.y, a[u].y);
-                        a[u].z = fmaf(v.w[t], x.z, a[u].z);
-                        a[u].w = fmaf(v.w[t], x.w, a[u].w);
+                    for (int u = 0; u < kB4; ++u) {
+                        const int f = f0 + 16 * u;
+                        if (f < nz4 && h0) {
+                            const float4 x = m[f];
+                            a[u].x = fmaf(wt, x.x, a[u].x); a[u].y = fmaf(wt, x.y, a[u].y);
+                            a[u].z = fmaf(wt, x.z, a[u].z); a[u].w = fmaf(wt, x.w, a[u].w);
+                        }
+                        if (f < nz4 && h1) {
+                            const float4 y = m[nz4 + f];
+                            b[u].x = fmaf(wt, y.x, b[u].x); b[u].y = fmaf(wt, y.y, b[u].y);
+                            b[u].z = fmaf(wt, y.z, b[u].z); b[u].w = fmaf(wt, y.w, b[u].w);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kB4; ++u) {
+                    const int f = f0 + 16 * u;
+                    if (f < nz4) {
+                        float2 *d = s + 4 * f * ld + q;
+                        d[0] = make_float2(a[u].x, b[u].x);
+                        d[ld] = make_float2(a[u].y, b[u].y);
+                        d[2 * ld] = make_float2(a[u].z, b[u].z);
+                        d[3 * ld] = make_float2(a[u].w, b[u].w);
                     }
                 }
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (lnv[u] >= 0) {
-                    float *col = reinterpret_cast<float *>(&s[(4 * fv[u]) * ld + (lnv[u] >> 1)]) + (lnv[u] & 1);
-                    col[0] = a[u].x;
-                    col[2 * ld] = a[u].y;
-                    col[4 * ld] = a[u].z;
-                    col[6 * ld] = a[u].w;
-                }
-            }
         }
-    } else {
-        const int total = nreal * Nz;
-        const float inv_nz = 1.0f / (float)Nz;
-        for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
-            float a[8];
+    } else for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
+        const int q = w >> 4, t = w & 15;
+        const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
+        const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
+        for (int z0 = t; z0 < Nz; z0 += 16 * kZBatch) {
+            float2 a[kZBatch];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * blockDim.x;
-                a[u] = (e < total) ? v.w[0] * v.act[0][base + e] : 0.f;
+            for (int u = 0; u < kZBatch; ++u) {
+                const int z = z0 + 16 * u;
+                a[u].x = (z < Nz && h0) ? v.w[0] * v.act[0][o0 + z] : 0.f;
+                a[u].y = (z < Nz && h1) ? v.w[0] * v.act[0][o0 + Nz + z] : 0.f;
             }
-            for (int t = 1; t < v.T; ++t) {
+            for (int tt = 1; tt < v.T; ++tt) {
+                const float *m = v.act[tt] + o0;
+                const float wt = v.w[tt];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = e0 + u * blockDim.x;
-                    if (e < total) a[u] = fmaf(v.w[t], v.act[t][base + e], a[u]);
+                for (int u = 0; u < kZBatch; ++u) {
+                    const int z = z0 + 16 * u;
+                    if (z < Nz && h0) a[u].x = fmaf(wt, m[z], a[u].x);
+                    if (z < Nz && h1) a[u].y = fmaf(wt, m[Nz + z], a[u].y);
                 }
             }
+            float2 *d = s + z0 * ld + q;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * blockDim.x;
-                if (e < total) {
-                    const int ln = fast_div(e, Nz, inv_nz);
-                    reinterpret_cast<float *>(&s[(e - ln * Nz) * ld + (ln >> 1)])[ln & 1] = a[u];
-                }
-            }
+            for (int u = 0; u < kZBatch; ++u)
+                if (z0 + 16 * u < Nz) d[16 * u * ld] = a[u];
         }
     }
     __syncthreads();
     z_fft<false>(s, lnl, ld, az, tw);
-    // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i)
-    const float inv_h = 1.0f / (float)H;
-    for (int e = threadIdx.x; e < nreal * H; e += blockDim.x) {
-        const int ln = fast_div(e, H, inv_h);
-        const int k = e - ln * H;
-        const int q = ln >> 1;
-        const float2 zk = s[zrow(az, k) * ld + q];
-        const float2 zm = s[zrow(az, k == 0 ? 0 : Pz - k) * ld + q];
-        const float2 r = (ln & 1) ? make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x))
-                                  : make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-        v.work[lineoff[ln] + k] = r;
+    // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i),
+    // k = t + 16 j; index P - k has row zrow(P - t) - j * step (zrow(P) is B or P)
+    const int rs = zrow_step(az) * ld;
+    for (int w = threadIdx.x; w < npair * 16; w += blockDim.x) {
+        const int q = w >> 4, t = w & 15;
+        if (t >= H) continue;
+        const bool h1 = 2 * q + 1 < nreal;
+        const float2 *pk = s + zrow(az, t) * ld + q;
+        const float2 *pm = s + zrow(az, Pz - t) * ld + q;
+        float2 *g0 = v.work + lineoff[2 * q] + t;
+        float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0) + t;
+        for (int k = t, j = 0; k < H; k += 16, ++j) {
+            const float2 zk = pk[j * rs];
+            float2 zm = zk;
+            if (k != 0) zm = pm[-j * rs];
+            g0[16 * j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            if (h1) g1[16 * j] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+        }
     }
 }
 
@@ -520,14 +526,6 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 
 // ---------------------------------------------------------------------------------
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
-__device__ __forceinline__ float iz_value(const float2 *s, const pvd_axis_d &az, int ld, int o, int ln) {
-    const int q = ln >> 1;
-    float val = reinterpret_cast<const float *>(&s[zrow(az, wrap_add(o, az.off, az.n)) * ld + q])[ln & 1];
-    if (o < az.foldn)
-        val += reinterpret_cast<const float *>(&s[zrow(az, (o + az.N + az.off) % az.n) * ld + q])[ln & 1];
-    return val;
-}
-
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
@@ -541,86 +539,77 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     load_twiddles(tw, az);
     tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
     __syncthreads();
-    // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) on the rest
-    const float inv_h = 1.0f / (float)H;
-    const int total = npair * H;
-    for (int e0 = threadIdx.x; e0 < total; e0 += 8 * blockDim.x) {
-        float2 A[8], B[8];
-        int qv[8], kv[8];
+    // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) for k in [1, P - H];
+    // pairs past the last line get zeros
+    for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
+        const int q = w >> 4, t = w & 15;
+        const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
+        const float2 *g0 = v.work + (h0 ? lineoff[2 * q] : 0);
+        const float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0);
+        for (int k0 = t; k0 < H; k0 += 16 * kZBatch) {
+            float2 A[kZBatch], B[kZBatch];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * blockDim.x;
-            const int q = fast_div(e, H, inv_h);
-            qv[u] = e < total ? q : -1;
-            kv[u] = e - q * H;
-            A[u] = make_float2(0.f, 0.f);
-            B[u] = make_float2(0.f, 0.f);
-            if (e < total) {
-                A[u] = v.work[lineoff[2 * q] + kv[u]];
-                if (2 * q + 1 < nreal) B[u] = v.work[lineoff[2 * q + 1] + kv[u]];
+            for (int u = 0; u < kZBatch; ++u) {
+                const int k = k0 + 16 * u;
+                A[u] = (k < H && h0) ? g0[k] : make_float2(0.f, 0.f);
+                B[u] = (k < H && h1) ? g1[k] : make_float2(0.f, 0.f);
             }
-        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (qv[u] >= 0) {
-                const int q = qv[u], k = kv[u];
-                s[k * ld + q] = make_float2(A[u].x - B[u].y, A[u].y + B[u].x);
-                if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
+            for (int u = 0; u < kZBatch; ++u) {
+                const int k = k0 + 16 * u;
+                if (k < H) {
+                    s[k * ld + q] = make_float2(A[u].x - B[u].y, A[u].y + B[u].x);
+                    if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
+                }
             }
-        }
-    }
-    if (npair < nl) {
-        const int unused = nl - npair;
-        for (int e = threadIdx.x; e < Pz * unused; e += blockDim.x) {
-            const int k = e / unused;
-            s[k * ld + npair + (e - k * unused)] = make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
     z_fft<true>(s, lnl, ld, az, tw);
-    const long long obase = L0 * Nz;
+    // epilogue: output o of real lines 2q and 2q+1 is the real and imaginary part of
+    // index i = (o + off) mod P of complex line q (plus index (o + N + off) mod P when
+    // o < foldn), times scale * rho_w/rho.  For o = t + 16 j the row of i advances by
+    // zrow_step per j and drops by zrow_wrap once i has wrapped past P.
+    const int rs = zrow_step(az) * ld, rw = zrow_wrap(az) * ld;
     const bool has_dens = v.dens != nullptr;
     const float sc = v.scale;
-    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps
-        const ZVec zv(nreal, Nz);
-        const float inv = 1.0f / (float)zv.nreal16;
-        constexpr int kEpi = 4;    // density loads in flight per thread
-        for (int i0 = threadIdx.x; i0 < zv.items; i0 += kEpi * blockDim.x) {
-            float4 d[kEpi];
-            int lnv[kEpi], fv[kEpi];
+    for (int w = threadIdx.x; w < npair * 16; w += blockDim.x) {
+        const int q = w >> 4, t = w & 15;
+        if (t >= Nz) continue;
+        const bool h1 = 2 * q + 1 < nreal;
+        const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
+        const int i0 = (t + az.off) % Pz;
+        const float2 *p = s + zrow(az, i0) * ld + q;
+        const int jw = (Pz - i0 + 15) >> 4;             // first j with i0 + 16 j >= P
+        for (int oo = t; oo < Nz; oo += 16 * kZBatch) {
+            const int jb = (oo - t) >> 4;
+            float2 d[kZBatch];
 #pragma unroll
-            for (int u = 0; u < kEpi; ++u) {
-                int ln, f;
-                const bool ok = zv.decode(i0 + u * blockDim.x, inv, ln, f);
-                lnv[u] = ok ? ln : -1;
-                fv[u] = f;
-                d[u] = make_float4(1.f, 1.f, 1.f, 1.f);
-                if (ok && has_dens) d[u] = reinterpret_cast<const float4 *>(v.dens + obase + (long long)ln * Nz)[f];
+            for (int u = 0; u < kZBatch; ++u) {
+                const int o = oo + 16 * u;
+                d[u].x = (has_dens && o < Nz) ? v.dens[o0 + o] : 1.f;
+                d[u].y = (has_dens && o < Nz && h1) ? v.dens[o0 + Nz + o] : 1.f;
             }
 #pragma unroll
-            for (int u = 0; u < kEpi; ++u) {
-                if (lnv[u] < 0) continue;
-                const int ln = lnv[u], o = 4 * fv[u];
-                float4 res = make_float4(iz_value(s, az, ld, o, ln) * sc, iz_value(s, az, ld, o + 1, ln) * sc,
-                                         iz_value(s, az, ld, o + 2, ln) * sc, iz_value(s, az, ld, o + 3, ln) * sc);
-                if (has_dens) {
-                    res.x *= density_factor(d[u].x);
-                    res.y *= density_factor(d[u].y);
-                    res.z *= density_factor(d[u].z);
-                    res.w *= density_factor(d[u].w);
+            for (int u = 0; u < kZBatch; ++u) {
+                const int o = oo + 16 * u;
+                if (o < Nz) {
+                    const int j = jb + u;
+                    float2 val = p[j * rs - (j >= jw ? rw : 0)];
+                    if (o < az.foldn) {
+                        const float2 f = s[zrow(az, (o + az.N + az.off) % Pz) * ld + q];
+                        val.x += f.x;
+                        val.y += f.y;
+                    }
+                    float f0 = sc, f1 = sc;
+                    if (has_dens) {
+                        f0 *= density_factor(d[u].x);
+                        f1 *= density_factor(d[u].y);
+                    }
+                    v.out[o0 + o] = val.x * f0;
+                    if (h1) v.out[o0 + Nz + o] = val.y * f1;
                 }
-                reinterpret_cast<float4 *>(v.out + obase + (long long)ln * Nz)[fv[u]] = res;
             }
-        }
-    } else {
-        const int nout = nreal * Nz;
-        const float inv_nz = 1.0f / (float)Nz;
-        for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-            const int ln = fast_div(e, Nz, inv_nz);
-            const float val = iz_value(s, az, ld, e - ln * Nz, ln);
-            float f = sc;
-            if (has_dens) f *= density_factor(v.dens[obase + e]);
-            v.out[obase + e] = val * f;
         }
     }
 }
