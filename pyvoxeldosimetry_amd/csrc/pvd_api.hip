@@ -324,7 +324,10 @@ int fft_setup(pvd_plan *p) {
         if (rc) return rc;
         v.ax[a].tw = tw;
     }
-    const size_t work_elems = (size_t)Nx * Py * v.Hp;
+    // x-planes of the workspace are Py * Hp complex plus PVD_PLANE_PAD (0: padding by one
+    // or three 128-byte lines measured the same on the CT workload)
+    v.wplane = Py * v.Hp + PVD_PLANE_PAD;
+    const size_t work_elems = (size_t)Nx * v.wplane;
     const size_t spec_elems = (size_t)Px * Py * v.Hp;
     PVD_HIP(hipMalloc(&p->work, sizeof(float2) * work_elems), "allocate FFT workspace");
     PVD_HIP(hipMalloc(&p->spec, sizeof(float2) * spec_elems), "allocate kernel spectrum");

@@ -93,12 +93,12 @@ __device__ __forceinline__ void store_col_tile(float2 *__restrict__ g, long long
 
 // ---------------------------------------------------------------------------------
 // Offsets in the workspace of the (up to 2 * PVD_TILE_MAX) real z-lines of a z-pass tile.
-__device__ __forceinline__ void tile_line_offsets(long long *lineoff, long long L0, int nreal, int Ny, int Py,
+__device__ __forceinline__ void tile_line_offsets(long long *lineoff, long long L0, int nreal, int Ny, int wplane,
                                                   int Hp) {
     for (int i = threadIdx.x; i < nreal; i += blockDim.x) {
         const int L = (int)(L0 + i);
         const int x = fast_div(L, Ny, 1.0f / (float)Ny);
-        lineoff[i] = ((long long)x * Py + (L - x * Ny)) * Hp;
+        lineoff[i] = (long long)x * wplane + (long long)(L - x * Ny) * Hp;
     }
 }
 
@@ -135,7 +135,7 @@ __device__ __forceinline__ void zreg_fft(float2 *__restrict__ s, const float2 *_
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     {
         const int w = threadIdx.x;
         const int k1 = w >> 4, q = w & 15;
@@ -149,7 +149,7 @@ __device__ __forceinline__ void zreg_fft(float2 *__restrict__ s, const float2 *_
             for (int k2 = 0; k2 < B; ++k2) p[k2 * LD] = v[k2];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Row of LDS that holds transform index k after z_fft.
@@ -200,7 +200,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     const int npair = (nreal + 1) >> 1;
     load_twiddles(tw, az);
-    tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
+    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
     // zero the padding rows z >= Nz of every line
     for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
     // weighted sum of the T maps into rows z < Nz; pairs past the last line get zeros
@@ -282,7 +282,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                 if (z0 + 16 * u < Nz) d[16 * u * ld] = a[u];
         }
     }
-    __syncthreads();
+    lds_barrier();
     z_fft<false>(s, lnl, ld, az, tw);
     // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i),
     // k = t + 16 j; index P - k has row zrow(P - t) - j * step (zrow(P) is B or P)
@@ -342,10 +342,23 @@ __device__ __forceinline__ void sq_transform(float2 *v, int l, int g, const floa
     sq_first<INV>(v, g, tw);
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) s[k1 * kSqPlane + g * 16 + l] = v[k1];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) v[n2] = s[g * kSqPlane + n2 * 16 + l];
     Dft<16, INV>::run(v);
+}
+
+// Element at a 32-bit byte offset from a wave-uniform base: the load or store takes
+// the base in scalar registers and one VGPR of offset (no 64-bit address math).
+// The 256-point passes address at most Nx * wplane * 8 <= 256 * (256 * 1032 + pad) * 8
+// bytes of workspace and 129 * 129 * 1032 * 4 bytes of spectrum, well below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
 template <bool INV>
@@ -356,27 +369,32 @@ __device__ void pass_y_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int x = tile / ntc;
     const int c0 = (tile - x * ntc) << 4;
     const int nc = min(16, H - c0);
-    float2 *g = v.work + ((long long)x * 256) * Hp + c0;
+    float2 *g = v.work + (long long)x * v.wplane + c0;    // uniform base of the tile
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    load_twiddles(tw, ay);
+    const float2 tw1 = ay.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
     const int rows_in = INV ? 256 : Ny;
+    const uint32_t lo = (uint32_t)(grp * Hp + l) * 8u, rstep = (uint32_t)Hp * 128u;   // + 16 rows
+    // every row r < 256 = Py and column c0 + l < Hp lies in the workspace: the loads
+    // are unconditional (no branches around them) and masked afterwards
     float2 a[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1) {
         const int r = 16 * n1 + grp;
-        a[n1] = (r < rows_in && l < nc) ? g[(long long)r * Hp + l] : make_float2(0.f, 0.f);
+        const float2 t = at((const float2 *)g, lo + n1 * rstep);
+        a[n1] = (r < rows_in && l < nc) ? t : make_float2(0.f, 0.f);
     }
-    __syncthreads();                                   // twiddles visible
+    tw[threadIdx.x] = tw1;
+    lds_barrier();                                     // twiddles visible
     sq_transform<INV>(a, l, grp, tw, s);
     if (l >= nc) return;
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
         const int k = grp + 16 * k2;
         if (!INV) {
-            g[(long long)k * Hp + l] = a[k2];
+            at(g, lo + k2 * rstep) = a[k2];
         } else {
             const int o = k >= ay.off ? k - ay.off : k + 256 - ay.off;     // output row reading index k
-            if (o < Ny) g[(long long)o * Hp + l] = a[k2];
+            if (o < Ny) at(g, (uint32_t)(o * Hp + l) * 8u) = a[k2];
         }
     }
 }
@@ -398,47 +416,57 @@ __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int y = x_tile_row(u, Py);
     const int c0 = (tile - u * ntc) << 4;
     const int nc = min(16, H - c0);
-    const long long plane = (long long)Py * Hp;
-    float2 *g = v.work + (long long)y * Hp + c0;
+    float2 *g = v.work + (long long)y * Hp + c0;       // uniform base; x-planes wplane apart
+    const uint32_t pstride = (uint32_t)v.wplane * 8u;  // bytes per x-plane
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    load_twiddles(tw, ax);
+    const float2 tw1 = ax.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
+    // loads are unconditional (rows clamped to Nx - 1, columns c0 + l < Hp lie in the
+    // row padding) and masked afterwards, so no branch skips them
     float2 a[16];
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) {
-        const int r = 16 * n1 + grp;
-        a[n1] = (r < Nx && l < nc) ? g[r * plane + l] : make_float2(0.f, 0.f);
-    }
-    // kernel spectrum of rows k = grp + 16 k2 (read before the barrier, in flight with the FFT)
+    for (int n1 = 0; n1 < 16; ++n1)
+        a[n1] = at((const float2 *)g, (uint32_t)min(16 * n1 + grp, Nx - 1) * pstride + (uint32_t)l * 8u);
+    // kernel spectrum of rows k = grp + 16 k2, in flight through the forward transform;
+    // columns l >= nc (row padding) multiply whatever the padding holds into columns
+    // that are never stored, and no column mixes with another
     float2 sp[16];
     if (v.sym) {
-        const float *sr = reinterpret_cast<const float *>(v.spec);
-        const long long pyh = Py / 2 + 1;
-        const long long row0 = (long long)min(y, Py - y) * Hp + c0 + l;
+        // real octant [Px/2+1][Py/2+1][Hp] FP32: row min(k, 256-k), column (min(y, Py-y), c0 + l)
+        const float *sr = reinterpret_cast<const float *>(v.spec) + (long long)min(y, Py - y) * Hp + c0;
+        const uint32_t krow = (uint32_t)(Py / 2 + 1) * (uint32_t)Hp * 4u;    // bytes per kx row
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const int k = grp + 16 * k2;
-            sp[k2] = make_float2(l < nc ? sr[(long long)min(k, 256 - k) * pyh * Hp + row0] : 0.f, 0.f);
+            sp[k2] = make_float2(at(sr, (uint32_t)min(k, 256 - k) * krow + (uint32_t)l * 4u), 0.f);
         }
     } else {
+        const float2 *sg = v.spec + (long long)y * Hp + c0;
+        const uint32_t kplane = (uint32_t)Py * (uint32_t)Hp * 8u;          // bytes per kx plane
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const int k = grp + 16 * k2;
-            sp[k2] = l < nc ? v.spec[(long long)y * Hp + c0 + k * plane + l] : make_float2(0.f, 0.f);
+            sp[k2] = at(sg, (uint32_t)k * kplane + (uint32_t)l * 8u);
         }
     }
-    __syncthreads();                                   // twiddles visible
+    // all 33 loads are issued before any result is consumed
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1)
+        if (!(16 * n1 + grp < Nx && l < nc)) a[n1] = make_float2(0.f, 0.f);   // zero rows past Nx
+    tw[threadIdx.x] = tw1;
+    lds_barrier();                                     // twiddles visible
     sq_transform<false>(a, l, grp, tw, s);             // a[k2] = X[grp + 16 k2]
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) a[k2] = v.sym ? cscale(a[k2], sp[k2].x) : cmul(a[k2], sp[k2]);
     // inverse: thread-local 16-point DFT over k2 and twiddle W256^(-j1 k1), k1 = grp
-    __syncthreads();                                   // every thread done reading s
+    lds_barrier();                                     // every thread done reading s
     sq_transform<true>(a, l, grp, tw, s);              // a[j2] = x[grp + 16 j2]
     if (l >= nc) return;
 #pragma unroll
     for (int j2 = 0; j2 < 16; ++j2) {
         const int j = grp + 16 * j2;
         const int o = j >= ax.off ? j - ax.off : j + 256 - ax.off;
-        if (o < Nx) g[o * plane + l] = a[j2];
+        if (o < Nx) at(g, (uint32_t)o * pstride + (uint32_t)l * 8u) = a[j2];
     }
 }
 
@@ -453,10 +481,10 @@ __device__ void pass_y(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int x = tile / ntc;
     const int c0 = (tile - x * ntc) << lnl;
     const int nc = min(TK, H - c0);
-    float2 *g = v.work + ((long long)x * Py) * Hp + c0;
+    float2 *g = v.work + (long long)x * v.wplane + c0;
     load_twiddles(tw, ay);
     load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
-    __syncthreads();
+    lds_barrier();
     fft_lines<INV>(s, lnl, TK, ay, tw);
     if (!INV) store_col_tile(g, Hp, s, Py, lnl, Py, 0, 0, nc);
     else store_col_tile(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
@@ -474,13 +502,14 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int y = x_tile_row(u, v.ax[1].n);
     const int c0 = (tile - u * ntc) << lnl;
     const int nc = min(TK, H - c0);
-    const long long plane = (long long)v.ax[1].n * Hp;
+    const long long plane = v.wplane;                 // work
+    const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
     const float2 *sp = v.spec + off0;
     load_twiddles(tw, ax);
     load_col_tile(s, g, plane, Px, Nx, lnl, nc);
-    __syncthreads();
+    lds_barrier();
     fft_lines<false>(s, lnl, TK, ax, tw);
     const int total = Px << lnl;
     if (v.sym) {
@@ -510,7 +539,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
                 const int k = e >> lnl, l = e & (TK - 1);
-                sv[u] = (e < total && l < nc) ? sp[k * plane + l] : make_float2(0.f, 0.f);
+                sv[u] = (e < total && l < nc) ? sp[k * splane + l] : make_float2(0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
@@ -519,7 +548,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     fft_lines<true>(s, lnl, TK, ax, tw);
     store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
 }
@@ -537,8 +566,8 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nreal = (int)min((long long)(2 * nl), nlines - L0);
     const int npair = (nreal + 1) >> 1;
     load_twiddles(tw, az);
-    tile_line_offsets(lineoff, L0, nreal, Ny, Py, Hp);
-    __syncthreads();
+    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
+    lds_barrier();
     // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) for k in [1, P - H];
     // pairs past the last line get zeros
     for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
@@ -564,7 +593,7 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     z_fft<true>(s, lnl, ld, az, tw);
     // epilogue: output o of real lines 2q and 2q+1 is the real and imaginary part of
     // index i = (o + off) mod P of complex line q (plus index (o + N + off) mod P when

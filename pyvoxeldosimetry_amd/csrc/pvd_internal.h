@@ -18,6 +18,11 @@
 // through a [16][16*16 + PVD_SQ256_PAD] float2 LDS array, PVD_SQ256_PAD x 16 float2
 // more than a plain 16-column tile of 256 rows.
 #define PVD_SQ256_PAD 16
+
+// Padding (complex elements) between x-planes of the FFT workspace.
+#ifndef PVD_PLANE_PAD
+#define PVD_PLANE_PAD 0
+#endif
 #define PVD_SQ256_EXTRA_LDS (16 * PVD_SQ256_PAD * 8)
 
 // One FFT axis of one volume.
@@ -56,14 +61,14 @@ struct pvd_vol_d {
     int ltkx;                         // log2 of columns per x-pass tile
     int vec_io;                       // 1: act/dens/out are 16-byte aligned (float4 access)
     int sym;                          // 1: spec holds the real octant [Px/2+1][Py/2+1][Hp] (FP32)
-    int pad3_;
+    int wplane;                       // complex elements per x-plane of work (Py*Hp + padding)
     const float *act[PVD_MAX_TIMEPOINTS];
     float w[PVD_MAX_TIMEPOINTS];
     const float *dens;
     float *out;
     float scale;
     int pad2_;
-    float2 *work;                     // [Nx][Py][Hp] complex workspace
+    float2 *work;                     // [Nx][Py][Hp] complex workspace, x-planes wplane apart
     const float2 *spec;               // [Px][Py][Hp] kernel spectrum (1/P folded in); see sym
     int tile_base[PVD_NPASS_FFT];     // first grid tile of this volume per pass (batch)
     int tiles[PVD_NPASS_FFT];         // tiles of this volume per pass
@@ -73,6 +78,18 @@ struct pvd_batch_args {
     const pvd_vol_d *vols;
     int nvol;
 };
+
+// ---------------------------------------------------------------------------------
+// Workgroup barrier that orders LDS only.  __syncthreads() carries a release fence
+// for all memory, which waits for every outstanding global load (vmcnt(0)) before
+// the barrier; the FFT passes exchange data only through LDS and consume every
+// global load before its value is shared, so their barriers wait for LDS
+// (lgkmcnt) alone and leave prefetched global loads in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // ---------------------------------------------------------------------------------
 // complex helpers
@@ -345,7 +362,7 @@ __device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int lnl, 
             dst[it] = (q * Ns * R + k) * ld + l;
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         if (dst[it] >= 0) {
@@ -353,7 +370,7 @@ __device__ __forceinline__ void stockham_stage(float2 *__restrict__ s, int lnl, 
             for (int r = 0; r < R; ++r) s[dst[it] + r * Ns * ld] = v[it][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Full FFT of length ax.n over 2^lnl lines in LDS (natural order in and out,
