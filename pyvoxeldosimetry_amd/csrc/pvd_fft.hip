@@ -158,8 +158,10 @@ __device__ __forceinline__ int zrow(const pvd_axis_d &az, int k) {
 }
 
 // z-axis FFT over the lines of a z-pass tile: the register two-step where the
-// plan selected one (16 lines per tile), else the LDS Stockham stages.
-template <bool INV>
+// plan selected one (16 lines per tile), else the LDS Stockham stages.  RZ
+// kernels (launched only for az.reg != 0) leave the Stockham code out, which
+// takes the z passes from 106 to 75 VGPRs.
+template <bool INV, bool RZ>
 __device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis_d &az, const float2 *tw) {
     switch (az.reg) {
         case 1: zreg_fft<8, 16, INV>(s, tw); break;
@@ -170,7 +172,9 @@ __device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis
         case 6: zreg_fft<8, 30, INV>(s, tw); break;
         case 7: zreg_fft<16, 20, INV>(s, tw); break;
         case 8: zreg_fft<16, 25, INV>(s, tw); break;
-        default: fft_lines<INV>(s, lnl, ld, az, tw); break;
+        default:
+            if (!RZ) fft_lines<INV>(s, lnl, ld, az, tw);
+            break;
     }
 }
 
@@ -189,6 +193,7 @@ constexpr int kZBatch = 8;      // elements per line in flight per thread
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
 // holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1),
 // real line 2q in the real and 2q+1 in the imaginary part of complex line q.
+template <bool RZ>
 __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
@@ -283,7 +288,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         }
     }
     lds_barrier();
-    z_fft<false>(s, lnl, ld, az, tw);
+    z_fft<false, RZ>(s, lnl, ld, az, tw);
     // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i),
     // k = t + 16 j; index P - k has row zrow(P - t) - j * step (zrow(P) is B or P)
     const int rs = zrow_step(az) * ld;
@@ -555,6 +560,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 
 // ---------------------------------------------------------------------------------
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
+template <bool RZ>
 __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     __shared__ long long lineoff[2 * PVD_TILE_MAX];
     const pvd_axis_d &az = v.ax[2];
@@ -594,7 +600,7 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         }
     }
     lds_barrier();
-    z_fft<true>(s, lnl, ld, az, tw);
+    z_fft<true, RZ>(s, lnl, ld, az, tw);
     // epilogue: output o of real lines 2q and 2q+1 is the real and imaginary part of
     // index i = (o + off) mod P of complex line q (plus index (o + N + off) mod P when
     // o < foldn), times scale * rho_w/rho.  For o = t + 16 j the row of i advances by
@@ -643,13 +649,13 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     }
 }
 
-template <int PASS>
+template <int PASS, bool RZ>
 __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     extern __shared__ float2 smem[];
     const int axis = (PASS == PVD_PASS_X) ? 0 : (PASS == PVD_PASS_FY || PASS == PVD_PASS_IY) ? 1 : 2;
     float2 *tw = smem;
     float2 *s = smem + ((v.ax[axis].n + 7) & ~7);
-    if (PASS == PVD_PASS_FZ) pass_fz(v, tile, tw, s);
+    if (PASS == PVD_PASS_FZ) pass_fz<RZ>(v, tile, tw, s);
     else if (PASS == PVD_PASS_FY) {
         if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<false>(v, tile, tw, s);
         else pass_y<false>(v, tile, tw, s);
@@ -659,13 +665,14 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     } else if (PASS == PVD_PASS_IY) {
         if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<true>(v, tile, tw, s);
         else pass_y<true>(v, tile, tw, s);
-    } else pass_iz(v, tile, tw, s);
+    } else pass_iz<RZ>(v, tile, tw, s);
 }
 
 }  // namespace
 
-// single volume: descriptor by value (kernarg segment, scalar loads)
-template <int PASS>
+// single volume: descriptor by value (kernarg segment, scalar loads); RZ: the z
+// passes of a plan with a register z transform
+template <int PASS, bool RZ = false>
 __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
@@ -675,11 +682,11 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
         tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
-    run_pass<PASS>(v, tile);
+    run_pass<PASS, RZ>(v, tile);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
-template <int PASS>
+template <int PASS, bool RZ = false>
 __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch_args a) {
     const int b = blockIdx.x;
     int lo = 0, hi = a.nvol - 1;
@@ -689,24 +696,33 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch
     }
     const pvd_vol_d &v = a.vols[lo];
     const int tile = b - v.tile_base[PASS];
-    if (tile < v.tiles[PASS]) run_pass<PASS>(v, tile);
+    if (tile < v.tiles[PASS]) run_pass<PASS, RZ>(v, tile);
 }
 
 // host-side launchers (called from pvd_api.hip)
 hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds,
                                hipStream_t st) {
+    const bool rz = v.ax[2].reg != 0;
     switch (pass) {
-        case PVD_PASS_FZ: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FZ>, dim3(grid), dim3(block), lds, st, v); break;
+        case PVD_PASS_FZ:
+            if (rz) hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_FZ, true>), dim3(grid), dim3(block), lds, st, v);
+            else hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_FZ, false>), dim3(grid), dim3(block), lds, st, v);
+            break;
         case PVD_PASS_FY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, v); break;
         case PVD_PASS_X:   // grid rounded up to a multiple of 8 (XCD bands; surplus blocks exit)
             hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_X>, dim3((grid + 7) & ~7), dim3(block), lds, st, v);
             break;
         case PVD_PASS_IY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, v); break;
-        default: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IZ>, dim3(grid), dim3(block), lds, st, v); break;
+        default:
+            if (rz) hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_IZ, true>), dim3(grid), dim3(block), lds, st, v);
+            else hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_IZ, false>), dim3(grid), dim3(block), lds, st, v);
+            break;
     }
     return hipGetLastError();
 }
 
+// (the batch kernels keep both z paths: reading the descriptor from the table
+// takes their register-z-only form to 119 VGPRs, above the generic 106)
 hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_batch_args &a, int grid, int block, size_t lds,
                                      hipStream_t st) {
     switch (pass) {
@@ -733,6 +749,8 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes) {
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_X>);
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_IY>);
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_IZ>);
+    PVD_SETATTR((pvd_fft_pass<PVD_PASS_FZ, true>));
+    PVD_SETATTR((pvd_fft_pass<PVD_PASS_IZ, true>));
 #undef PVD_SETATTR
     return e;
 }
