@@ -109,9 +109,21 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
             if (i < HALO) *reinterpret_cast<uint4 *>(hl + (i / C8) * VS + (i % C8) * 16) = pf[u];
         }
         __syncthreads();
-        if (b + (int)gridDim.x < total) fetch(b + gridDim.x);
         const int ox = x0 + wx, oy = y0 + wy;
-        if (ox >= a.n0 || oy >= a.n1) continue;      // wave-uniform
+        const bool live = ox < a.n0 && oy < a.n1;    // wave-uniform
+        const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
+        // skip term of a residual layer: a layer whose skip input is its own input
+        // (resid == x, 32 channels) reads it from the centre of the halo in LDS;
+        // any other tensor is loaded here, before the next halo, so the epilogue
+        // waits for these loads only
+        const bool resid_lds = EPI == PVD_CNN_EPI_RESID_RELU && CIN == 32 && a.resid == a.x;
+        uint16_t rs[16];
+        if (EPI == PVD_CNN_EPI_RESID_RELU && !resid_lds && live) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) rs[i] = a.resid[(vrow + mfma_row(i, lane)) * 32 + m];
+        }
+        if (b + (int)gridDim.x < total) fetch(b + gridDim.x);
+        if (!live) continue;
         // two independent accumulation chains (by tap parity), so consecutive MFMAs
         // of a wave do not wait for each other's results
         f32x16 acc0 = {}, acc1 = {};
@@ -130,7 +142,6 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
             }
         }
         const f32x16 acc = acc0 + acc1;
-        const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
         const int co = m;
         const float bco = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
@@ -144,7 +155,11 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
             } else if (EPI == PVD_CNN_EPI_RESID_RELU) {
                 const float av = fmaxf(zv, 0.0f);
                 if (a.act) a.act[v * 32 + co] = f2bf(av);
-                a.y[v * 32 + co] = f2bf(bf2f(a.resid[v * 32 + co]) + av);
+                uint16_t r = rs[i];
+                if (resid_lds)   // halo voxel (wx + 1, wy + 1, z + 1), channel co
+                    r = *reinterpret_cast<const uint16_t *>(
+                        hl + (((wx + 1) * HY + (wy + 1)) * HZ + mfma_row(i, lane) + 1) * VS + co * 2);
+                a.y[v * 32 + co] = f2bf(bf2f(r) + av);
             } else if (EPI == PVD_CNN_EPI_LINEAR_F32) {
                 if (co == 0) a.y32[v] = zv;
             } else {

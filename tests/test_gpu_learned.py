@@ -84,6 +84,14 @@ def test_conv_forward_all_epilogues(cin):
         y3 = torch.empty_like(y)
         _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, SHAPE, y=y3, resid=resid)
         assert torch.equal(y2, y3)
+        # skip input = the layer's own input: read from the halo in LDS, same bits as
+        # the same values passed as a separate tensor (global loads)
+        y4, y5 = torch.empty_like(y), torch.empty_like(y)
+        _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, SHAPE, y=y4, resid=xt)
+        _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, SHAPE, y=y5, resid=xt.clone())
+        assert torch.equal(y4, y5)
+        np.testing.assert_allclose(y4.float().cpu().numpy().reshape(SHAPE + (32,)), x + np.maximum(z, 0),
+                                   rtol=8e-3, atol=2e-3)
 
 
 @pytest.mark.parametrize("cin", [16, 32])
@@ -108,6 +116,13 @@ def test_conv_forward_many_blocks_per_workgroup(cin):
     y = torch.empty((n, 32), dtype=torch.bfloat16, device="cuda")
     _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RELU, shape, y=y)
     np.testing.assert_allclose(y.float().cpu().numpy().reshape(shape + (32,)), np.maximum(z, 0), rtol=8e-3, atol=1e-3)
+    if cin == 32:     # residual layer, skip term from the halo (resid == x) and from a copy
+        y4, y5 = torch.empty_like(y), torch.empty_like(y)
+        _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, shape, y=y4, resid=xt)
+        _conv(lib, xt, cin, wt, bt, nat.PVD_CNN_EPI_RESID_RELU, shape, y=y5, resid=xt.clone())
+        assert torch.equal(y4, y5)
+        np.testing.assert_allclose(y4.float().cpu().numpy().reshape(shape + (32,)), x + np.maximum(z, 0),
+                                   rtol=8e-3, atol=2e-3)
 
 
 @pytest.mark.parametrize("cin", [16, 32])
