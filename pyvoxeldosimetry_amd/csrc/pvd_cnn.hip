@@ -102,13 +102,16 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
     for (int b = blockIdx.x; b < total; b += gridDim.x) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
-        __syncthreads();   // weights visible / previous block finished with the halo
+        // LDS-only barriers: the previous block's output stores stay in flight (a
+        // full __syncthreads would wait for them); the prefetched halo registers are
+        // waited for where they are written
+        lds_barrier();     // weights visible / previous block finished with the halo
 #pragma unroll
         for (int u = 0; u < PER_T; ++u) {
             const int i = tid + u * FWD_THREADS;
             if (i < HALO) *reinterpret_cast<uint4 *>(hl + (i / C8) * VS + (i % C8) * 16) = pf[u];
         }
-        __syncthreads();
+        lds_barrier();
         const int ox = x0 + wx, oy = y0 + wy;
         const bool live = ox < a.n0 && oy < a.n1;    // wave-uniform
         const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
