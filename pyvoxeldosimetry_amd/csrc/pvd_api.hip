@@ -20,7 +20,7 @@
 // kernels (pvd_fft.hip, pvd_spectrum.hip, pvd_direct.hip)
 hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds, hipStream_t st);
 hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_batch_args &a, int grid, int block, size_t lds,
-                                     hipStream_t st);
+                                     hipStream_t st, int max_lines);
 hipError_t pvd_set_fft_lds_limits(size_t bytes);
 hipError_t pvd_launch_dpk(double v, const int g[3], const pvd_dpk_term *terms, int nterms, double factor,
                           double *out, hipStream_t st);
@@ -271,6 +271,10 @@ int fft_setup(pvd_plan *p) {
     v.H = Pz / 2 + 1;
     int lz, bz; size_t ldsz;
     pick_tile(Pz, p->tile[2], 1, &lz, &bz, &ldsz);
+    // z blocks are a power-of-two multiple of the lines per tile, so the z passes'
+    // I/O phases (blockDim / lines threads per line pair) keep every thread busy
+    // (at most 1024: both factors are powers of two)
+    { int per = 1; while (per * lz < bz) per *= 2; bz = per * lz; }
     int tky, by; size_t ldsy;
     pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
@@ -771,9 +775,11 @@ int batch_launch(pvd_batch *b, hipStream_t st) {
     pvd_batch_args a;
     a.vols = b->d_tab;
     a.nvol = b->nvol;
+    int max_lines = 1;
+    for (int i = 0; i < b->nvol; ++i) max_lines = std::max(max_lines, 1 << b->h_tab[i].llz);
     for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
         const launch_cfg &c = b->cfg[pass];
-        PVD_HIP(pvd_launch_fft_pass_batch(pass, a, c.grid, c.block, c.lds, st), "launch batch FFT pass");
+        PVD_HIP(pvd_launch_fft_pass_batch(pass, a, c.grid, c.block, c.lds, st, max_lines), "launch batch FFT pass");
     }
     return PVD_OK;
 }

@@ -178,49 +178,44 @@ __device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis
     }
 }
 
-// Row step of z_fft's result per +16 in the index, and the drop in rows where an
-// index wraps past n: 16 / A and B for the register two-step (row of k is
-// (k mod A) * B + k / A), 16 and n for the Stockham stages (row of k is k).
-__device__ __forceinline__ int zrow_step(const pvd_axis_d &az) { return az.reg ? 16 >> az.ra_log : 16; }
+// Row step of z_fft's result per +TP in the index, and the drop in rows where an
+// index wraps past n: TP / A and B for the register two-step (row of k is
+// (k mod A) * B + k / A; A divides TP), TP and n for the Stockham stages (row of
+// k is k).
+__device__ __forceinline__ int zrow_step(const pvd_axis_d &az, int tp) { return az.reg ? tp >> az.ra_log : tp; }
 __device__ __forceinline__ int zrow_wrap(const pvd_axis_d &az) { return az.reg ? az.rb : az.n; }
 
-// The z passes move the elements t + 16 j (j = 0, 1, ...) of one line pair per
-// thread (q, t), 16 threads per pair: global accesses are 16 consecutive words
-// per line, LDS rows advance by a constant stride (zrow_step), both real lines
-// of a pair travel in one float2, and no index is divided.
+// The z passes move the elements t + TP j (j = 0, 1, ...) of one line pair per
+// thread (q, t), TP threads per pair: 16, 32 or 64, the largest that blockDim
+// covers for the tile's lines (the plan sizes z blocks to a power-of-two multiple
+// of the lines; smaller blocks loop).  Global accesses are TP consecutive words
+// per line, LDS rows advance by a constant stride (zrow_step), both real lines of
+// a pair travel in one float2, and no index is divided.  A divides TP.
 constexpr int kZBatch = 8;      // elements per line in flight per thread
+template <int TP> constexpr int z_tpl() { return TP == 16 ? 4 : TP == 32 ? 5 : 6; }
 
 // FZ: lines are the Nx*Ny real z-lines in flattened order L = x*Ny + y; a tile
 // holds 2*nl consecutive real lines paired into nl complex lines (ld = nl + 1),
 // real line 2q in the real and 2q+1 in the imaginary part of complex line q.
-template <bool RZ>
-__device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[2 * PVD_TILE_MAX];
-    const pvd_axis_d &az = v.ax[2];
-    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
-    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
-    const int Ny = v.ax[1].N, Py = v.ax[1].n;
-    const long long nlines = (long long)v.ax[0].N * Ny;
-    const long long L0 = (long long)tile * 2 * nl;
-    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
-    const int npair = (nreal + 1) >> 1;
-    load_twiddles(tw, az);
-    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
-    // zero the padding rows z >= Nz of every line
-    for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
+// FZ phases with TP threads per line pair (see kZBatch): the T-weighted load of
+// rows z < Nz, and the split of the two real spectra into the workspace.
+template <int TP>
+__device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long L0, int nreal, int nl, int ld,
+                                        int Nz) {
+    constexpr int tpl = z_tpl<TP>();
     // weighted sum of the T maps into rows z < Nz; pairs past the last line get zeros
-    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + 16 j)
+    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + TP j)
         const int nz4 = Nz >> 2;
         constexpr int kB4 = kZBatch / 2;
-        for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
-            const int q = w >> 4, t = w & 15;
+        for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
+            const int q = w >> tpl, t = w & (TP - 1);
             const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
             const long long o0 = (L0 + 2 * q) * Nz;
-            for (int f0 = t; f0 < nz4; f0 += 16 * kB4) {
+            for (int f0 = t; f0 < nz4; f0 += TP * kB4) {
                 float4 a[kB4], b[kB4];
 #pragma unroll
                 for (int u = 0; u < kB4; ++u) {
-                    const int f = f0 + 16 * u;
+                    const int f = f0 + TP * u;
                     const float4 *m = reinterpret_cast<const float4 *>(v.act[0] + o0);
                     const float4 x = (f < nz4 && h0) ? m[f] : make_float4(0.f, 0.f, 0.f, 0.f);
                     const float4 y = (f < nz4 && h1) ? m[nz4 + f] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -233,7 +228,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                     const float wt = v.w[tt];
 #pragma unroll
                     for (int u = 0; u < kB4; ++u) {
-                        const int f = f0 + 16 * u;
+                        const int f = f0 + TP * u;
                         if (f < nz4 && h0) {
                             const float4 x = m[f];
                             a[u].x = fmaf(wt, x.x, a[u].x); a[u].y = fmaf(wt, x.y, a[u].y);
@@ -248,7 +243,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                 }
 #pragma unroll
                 for (int u = 0; u < kB4; ++u) {
-                    const int f = f0 + 16 * u;
+                    const int f = f0 + TP * u;
                     if (f < nz4) {
                         float2 *d = s + 4 * f * ld + q;
                         d[0] = make_float2(a[u].x, b[u].x);
@@ -259,15 +254,15 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                 }
             }
         }
-    } else for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
-        const int q = w >> 4, t = w & 15;
+    } else for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
+        const int q = w >> tpl, t = w & (TP - 1);
         const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
         const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
-        for (int z0 = t; z0 < Nz; z0 += 16 * kZBatch) {
+        for (int z0 = t; z0 < Nz; z0 += TP * kZBatch) {
             float2 a[kZBatch];
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
-                const int z = z0 + 16 * u;
+                const int z = z0 + TP * u;
                 a[u].x = (z < Nz && h0) ? v.w[0] * v.act[0][o0 + z] : 0.f;
                 a[u].y = (z < Nz && h1) ? v.w[0] * v.act[0][o0 + Nz + z] : 0.f;
             }
@@ -276,7 +271,7 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                 const float wt = v.w[tt];
 #pragma unroll
                 for (int u = 0; u < kZBatch; ++u) {
-                    const int z = z0 + 16 * u;
+                    const int z = z0 + TP * u;
                     if (z < Nz && h0) a[u].x = fmaf(wt, m[z], a[u].x);
                     if (z < Nz && h1) a[u].y = fmaf(wt, m[Nz + z], a[u].y);
                 }
@@ -284,30 +279,55 @@ __device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             float2 *d = s + z0 * ld + q;
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u)
-                if (z0 + 16 * u < Nz) d[16 * u * ld] = a[u];
+                if (z0 + TP * u < Nz) d[TP * u * ld] = a[u];
         }
     }
-    lds_barrier();
-    z_fft<false, RZ>(s, lnl, ld, az, tw);
+}
+
+template <int TP>
+__device__ __forceinline__ void fz_split(const pvd_vol_d &v, float2 *s, const long long *lineoff,
+                                         const pvd_axis_d &az, int npair, int nreal, int ld, int H, int Pz) {
+    constexpr int tpl = z_tpl<TP>();
     // split the two real spectra: A = (Z_k + conj Z_{-k})/2, B = (Z_k - conj Z_{-k})/(2i),
-    // k = t + 16 j; index P - k has row zrow(P - t) - j * step (zrow(P) is B or P)
-    const int rs = zrow_step(az) * ld;
-    for (int w = threadIdx.x; w < npair * 16; w += blockDim.x) {
-        const int q = w >> 4, t = w & 15;
+    // k = t + TP j; index P - k has row zrow(P - t) - j * step (zrow(P) is B or P)
+    const int rs = zrow_step(az, TP) * ld;
+    for (int w = threadIdx.x; w < npair * TP; w += blockDim.x) {
+        const int q = w >> tpl, t = w & (TP - 1);
         if (t >= H) continue;
         const bool h1 = 2 * q + 1 < nreal;
         const float2 *pk = s + zrow(az, t) * ld + q;
         const float2 *pm = s + zrow(az, Pz - t) * ld + q;
         float2 *g0 = v.work + lineoff[2 * q] + t;
         float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0) + t;
-        for (int k = t, j = 0; k < H; k += 16, ++j) {
+        for (int k = t, j = 0; k < H; k += TP, ++j) {
             const float2 zk = pk[j * rs];
             float2 zm = zk;
             if (k != 0) zm = pm[-j * rs];
-            g0[16 * j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            if (h1) g1[16 * j] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+            g0[TP * j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            if (h1) g1[TP * j] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
         }
     }
+}
+
+template <bool RZ, int TP>
+__device__ void pass_fz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    __shared__ long long lineoff[2 * PVD_TILE_MAX];
+    const pvd_axis_d &az = v.ax[2];
+    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
+    const int Ny = v.ax[1].N, Py = v.ax[1].n;
+    const long long nlines = (long long)v.ax[0].N * Ny;
+    const long long L0 = (long long)tile * 2 * nl;
+    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
+    const int npair = (nreal + 1) >> 1;
+    load_twiddles(tw, az);
+    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
+    // zero the padding rows z >= Nz of every line
+    for (int e = threadIdx.x; e < (Pz - Nz) * ld; e += blockDim.x) s[Nz * ld + e] = make_float2(0.f, 0.f);
+    fz_load<TP>(v, s, L0, nreal, nl, ld, Nz);
+    lds_barrier();
+    z_fft<false, RZ>(s, lnl, ld, az, tw);
+    fz_split<TP>(v, s, lineoff, az, npair, nreal, ld, H, Pz);
 }
 
 // ---------------------------------------------------------------------------------
@@ -560,38 +580,30 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 
 // ---------------------------------------------------------------------------------
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
-template <bool RZ>
-__device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
-    __shared__ long long lineoff[2 * PVD_TILE_MAX];
-    const pvd_axis_d &az = v.ax[2];
-    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
-    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
-    const int Ny = v.ax[1].N, Py = v.ax[1].n;
-    const long long nlines = (long long)v.ax[0].N * Ny;
-    const long long L0 = (long long)tile * 2 * nl;
-    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
-    const int npair = (nreal + 1) >> 1;
-    load_twiddles(tw, az);
-    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
-    lds_barrier();
+// IZ phases with TP threads per line pair: the half-spectrum load (both halves of
+// each complex line) and the epilogue (roll / fold, scale * rho_w/rho, store).
+template <int TP>
+__device__ __forceinline__ void iz_load(const pvd_vol_d &v, float2 *s, const long long *lineoff, int nl, int nreal,
+                                        int ld, int H, int Pz) {
     // Z_k = A_k + i B_k on [0, H); Z_{P-k} = conj(A_k) + i conj(B_k) for k in [1, P - H];
     // pairs past the last line get zeros
-    for (int w = threadIdx.x; w < nl * 16; w += blockDim.x) {
-        const int q = w >> 4, t = w & 15;
+    constexpr int tpl = z_tpl<TP>();
+    for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
+        const int q = w >> tpl, t = w & (TP - 1);
         const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
         const float2 *g0 = v.work + (h0 ? lineoff[2 * q] : 0);
         const float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0);
-        for (int k0 = t; k0 < H; k0 += 16 * kZBatch) {
+        for (int k0 = t; k0 < H; k0 += TP * kZBatch) {
             float2 A[kZBatch], B[kZBatch];
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
-                const int k = k0 + 16 * u;
+                const int k = k0 + TP * u;
                 A[u] = (k < H && h0) ? g0[k] : make_float2(0.f, 0.f);
                 B[u] = (k < H && h1) ? g1[k] : make_float2(0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
-                const int k = k0 + 16 * u;
+                const int k = k0 + TP * u;
                 if (k < H) {
                     s[k * ld + q] = make_float2(A[u].x - B[u].y, A[u].y + B[u].x);
                     if (k >= 1 && k <= Pz - H) s[(Pz - k) * ld + q] = make_float2(A[u].x + B[u].y, B[u].x - A[u].y);
@@ -599,35 +611,39 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             }
         }
     }
-    lds_barrier();
-    z_fft<true, RZ>(s, lnl, ld, az, tw);
+}
+
+template <int TP>
+__device__ __forceinline__ void iz_epilogue(const pvd_vol_d &v, const float2 *s, const pvd_axis_d &az, long long L0,
+                                            int npair, int nreal, int ld, int Nz, int Pz) {
+    constexpr int tpl = z_tpl<TP>();
     // epilogue: output o of real lines 2q and 2q+1 is the real and imaginary part of
     // index i = (o + off) mod P of complex line q (plus index (o + N + off) mod P when
-    // o < foldn), times scale * rho_w/rho.  For o = t + 16 j the row of i advances by
+    // o < foldn), times scale * rho_w/rho.  For o = t + TP j the row of i advances by
     // zrow_step per j and drops by zrow_wrap once i has wrapped past P.
-    const int rs = zrow_step(az) * ld, rw = zrow_wrap(az) * ld;
+    const int rs = zrow_step(az, TP) * ld, rw = zrow_wrap(az) * ld;
     const bool has_dens = v.dens != nullptr;
     const float sc = v.scale;
-    for (int w = threadIdx.x; w < npair * 16; w += blockDim.x) {
-        const int q = w >> 4, t = w & 15;
+    for (int w = threadIdx.x; w < npair * TP; w += blockDim.x) {
+        const int q = w >> tpl, t = w & (TP - 1);
         if (t >= Nz) continue;
         const bool h1 = 2 * q + 1 < nreal;
         const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
         const int i0 = (t + az.off) % Pz;
         const float2 *p = s + zrow(az, i0) * ld + q;
-        const int jw = (Pz - i0 + 15) >> 4;             // first j with i0 + 16 j >= P
-        for (int oo = t; oo < Nz; oo += 16 * kZBatch) {
-            const int jb = (oo - t) >> 4;
+        const int jw = (Pz - i0 + TP - 1) >> tpl;       // first j with i0 + TP j >= P
+        for (int oo = t; oo < Nz; oo += TP * kZBatch) {
+            const int jb = (oo - t) >> tpl;
             float2 d[kZBatch];
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
-                const int o = oo + 16 * u;
+                const int o = oo + TP * u;
                 d[u].x = (has_dens && o < Nz) ? v.dens[o0 + o] : 1.f;
                 d[u].y = (has_dens && o < Nz && h1) ? v.dens[o0 + Nz + o] : 1.f;
             }
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
-                const int o = oo + 16 * u;
+                const int o = oo + TP * u;
                 if (o < Nz) {
                     const int j = jb + u;
                     float2 val = p[j * rs - (j >= jw ? rw : 0)];
@@ -649,13 +665,33 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     }
 }
 
-template <int PASS, bool RZ>
+template <bool RZ, int TP>
+__device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    __shared__ long long lineoff[2 * PVD_TILE_MAX];
+    const pvd_axis_d &az = v.ax[2];
+    const int Nz = az.N, Pz = az.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.llz, nl = 1 << lnl, ld = nl + 1;
+    const int Ny = v.ax[1].N, Py = v.ax[1].n;
+    const long long nlines = (long long)v.ax[0].N * Ny;
+    const long long L0 = (long long)tile * 2 * nl;
+    const int nreal = (int)min((long long)(2 * nl), nlines - L0);
+    const int npair = (nreal + 1) >> 1;
+    load_twiddles(tw, az);
+    tile_line_offsets(lineoff, L0, nreal, Ny, v.wplane, Hp);
+    lds_barrier();
+    iz_load<TP>(v, s, lineoff, nl, nreal, ld, H, Pz);
+    lds_barrier();
+    z_fft<true, RZ>(s, lnl, ld, az, tw);
+    iz_epilogue<TP>(v, s, az, L0, npair, nreal, ld, Nz, Pz);
+}
+
+template <int PASS, bool RZ, int TP>
 __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     extern __shared__ float2 smem[];
     const int axis = (PASS == PVD_PASS_X) ? 0 : (PASS == PVD_PASS_FY || PASS == PVD_PASS_IY) ? 1 : 2;
     float2 *tw = smem;
     float2 *s = smem + ((v.ax[axis].n + 7) & ~7);
-    if (PASS == PVD_PASS_FZ) pass_fz<RZ>(v, tile, tw, s);
+    if (PASS == PVD_PASS_FZ) pass_fz<RZ, TP>(v, tile, tw, s);
     else if (PASS == PVD_PASS_FY) {
         if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<false>(v, tile, tw, s);
         else pass_y<false>(v, tile, tw, s);
@@ -665,14 +701,16 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     } else if (PASS == PVD_PASS_IY) {
         if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<true>(v, tile, tw, s);
         else pass_y<true>(v, tile, tw, s);
-    } else pass_iz<RZ>(v, tile, tw, s);
+    } else pass_iz<RZ, TP>(v, tile, tw, s);
 }
 
 }  // namespace
 
-// single volume: descriptor by value (kernarg segment, scalar loads); RZ: the z
-// passes of a plan with a register z transform
-template <int PASS, bool RZ = false>
+// single volume: descriptor by value (kernarg segment, scalar loads).  Only the z
+// passes use the template arguments: RZ for a plan with a register z transform,
+// TP threads per line pair in their I/O phases (the host picks both; one kernel
+// per combination, as branches between them inflate the register allocation).
+template <int PASS, bool RZ = false, int TP = 16>
 __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
@@ -682,11 +720,11 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
         tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
-    run_pass<PASS, RZ>(v, tile);
+    run_pass<PASS, RZ, TP>(v, tile);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
-template <int PASS, bool RZ = false>
+template <int PASS, int TP = 16>
 __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch_args a) {
     const int b = blockIdx.x;
     int lo = 0, hi = a.nvol - 1;
@@ -696,17 +734,40 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch
     }
     const pvd_vol_d &v = a.vols[lo];
     const int tile = b - v.tile_base[PASS];
-    if (tile < v.tiles[PASS]) run_pass<PASS, RZ>(v, tile);
+    if (tile < v.tiles[PASS]) run_pass<PASS, false, TP>(v, tile);
 }
 
 // host-side launchers (called from pvd_api.hip)
+namespace {
+// threads per line pair of the z passes' I/O: the largest of 64, 32, 16 with
+// tp * lines <= block (16 when the block is smaller; its threads then loop)
+int z_threads_per_pair(int block, int lines) {
+    return block >= 64 * lines ? 64 : block >= 32 * lines ? 32 : 16;
+}
+
+template <int PASS, bool RZ>
+void launch_z(int tp, int grid, int block, size_t lds, hipStream_t st, const pvd_vol_d &v) {
+    if (tp == 64) hipLaunchKernelGGL((pvd_fft_pass<PASS, RZ, 64>), dim3(grid), dim3(block), lds, st, v);
+    else if (tp == 32) hipLaunchKernelGGL((pvd_fft_pass<PASS, RZ, 32>), dim3(grid), dim3(block), lds, st, v);
+    else hipLaunchKernelGGL((pvd_fft_pass<PASS, RZ, 16>), dim3(grid), dim3(block), lds, st, v);
+}
+
+template <int PASS>
+void launch_z_batch(int tp, int grid, int block, size_t lds, hipStream_t st, const pvd_batch_args &a) {
+    if (tp == 64) hipLaunchKernelGGL((pvd_fft_pass_batch<PASS, 64>), dim3(grid), dim3(block), lds, st, a);
+    else if (tp == 32) hipLaunchKernelGGL((pvd_fft_pass_batch<PASS, 32>), dim3(grid), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL((pvd_fft_pass_batch<PASS, 16>), dim3(grid), dim3(block), lds, st, a);
+}
+}  // namespace
+
 hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds,
                                hipStream_t st) {
     const bool rz = v.ax[2].reg != 0;
+    const int tp = z_threads_per_pair(block, 1 << v.llz);
     switch (pass) {
         case PVD_PASS_FZ:
-            if (rz) hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_FZ, true>), dim3(grid), dim3(block), lds, st, v);
-            else hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_FZ, false>), dim3(grid), dim3(block), lds, st, v);
+            if (rz) launch_z<PVD_PASS_FZ, true>(tp, grid, block, lds, st, v);
+            else launch_z<PVD_PASS_FZ, false>(tp, grid, block, lds, st, v);
             break;
         case PVD_PASS_FY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, v); break;
         case PVD_PASS_X:   // grid rounded up to a multiple of 8 (XCD bands; surplus blocks exit)
@@ -714,8 +775,8 @@ hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block
             break;
         case PVD_PASS_IY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, v); break;
         default:
-            if (rz) hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_IZ, true>), dim3(grid), dim3(block), lds, st, v);
-            else hipLaunchKernelGGL((pvd_fft_pass<PVD_PASS_IZ, false>), dim3(grid), dim3(block), lds, st, v);
+            if (rz) launch_z<PVD_PASS_IZ, true>(tp, grid, block, lds, st, v);
+            else launch_z<PVD_PASS_IZ, false>(tp, grid, block, lds, st, v);
             break;
     }
     return hipGetLastError();
@@ -723,14 +784,16 @@ hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block
 
 // (the batch kernels keep both z paths: reading the descriptor from the table
 // takes their register-z-only form to 119 VGPRs, above the generic 106)
+// max_lines: the largest z tile (complex lines) of the batch's volumes
 hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_batch_args &a, int grid, int block, size_t lds,
-                                     hipStream_t st) {
+                                     hipStream_t st, int max_lines) {
+    const int tp = z_threads_per_pair(block, max_lines);
     switch (pass) {
-        case PVD_PASS_FZ: hipLaunchKernelGGL(pvd_fft_pass_batch<PVD_PASS_FZ>, dim3(grid), dim3(block), lds, st, a); break;
+        case PVD_PASS_FZ: launch_z_batch<PVD_PASS_FZ>(tp, grid, block, lds, st, a); break;
         case PVD_PASS_FY: hipLaunchKernelGGL(pvd_fft_pass_batch<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, a); break;
         case PVD_PASS_X: hipLaunchKernelGGL(pvd_fft_pass_batch<PVD_PASS_X>, dim3(grid), dim3(block), lds, st, a); break;
         case PVD_PASS_IY: hipLaunchKernelGGL(pvd_fft_pass_batch<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, a); break;
-        default: hipLaunchKernelGGL(pvd_fft_pass_batch<PVD_PASS_IZ>, dim3(grid), dim3(block), lds, st, a); break;
+        default: launch_z_batch<PVD_PASS_IZ>(tp, grid, block, lds, st, a); break;
     }
     return hipGetLastError();
 }
@@ -749,8 +812,14 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes) {
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_X>);
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_IY>);
     PVD_SETATTR(pvd_fft_pass_batch<PVD_PASS_IZ>);
-    PVD_SETATTR((pvd_fft_pass<PVD_PASS_FZ, true>));
-    PVD_SETATTR((pvd_fft_pass<PVD_PASS_IZ, true>));
+#define PVD_SETATTR_Z(P)                                                                        \
+    PVD_SETATTR((pvd_fft_pass<P, false, 32>)); PVD_SETATTR((pvd_fft_pass<P, false, 64>));         \
+    PVD_SETATTR((pvd_fft_pass<P, true, 16>)); PVD_SETATTR((pvd_fft_pass<P, true, 32>));           \
+    PVD_SETATTR((pvd_fft_pass<P, true, 64>));                                                     \
+    PVD_SETATTR((pvd_fft_pass_batch<P, 32>)); PVD_SETATTR((pvd_fft_pass_batch<P, 64>))
+    PVD_SETATTR_Z(PVD_PASS_FZ);
+    PVD_SETATTR_Z(PVD_PASS_IZ);
+#undef PVD_SETATTR_Z
 #undef PVD_SETATTR
     return e;
 }
