@@ -299,12 +299,26 @@ __device__ __forceinline__ void fz_split(const pvd_vol_d &v, float2 *s, const lo
         const float2 *pm = s + zrow(az, Pz - t) * ld + q;
         float2 *g0 = v.work + lineoff[2 * q] + t;
         float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0) + t;
-        for (int k = t, j = 0; k < H; k += TP, ++j) {
-            const float2 zk = pk[j * rs];
-            float2 zm = zk;
-            if (k != 0) zm = pm[-j * rs];
-            g0[TP * j] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            if (h1) g1[TP * j] = make_float2(0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x));
+        for (int k0 = t, j0 = 0; k0 < H; k0 += TP * kZBatch, j0 += kZBatch) {
+            float2 zk[kZBatch], zm[kZBatch];       // all LDS reads of the batch before any store
+#pragma unroll
+            for (int u = 0; u < kZBatch; ++u) {
+                const int k = k0 + TP * u, j = j0 + u;
+                zk[u] = make_float2(0.f, 0.f);
+                zm[u] = make_float2(0.f, 0.f);
+                if (k < H) {
+                    zk[u] = pk[j * rs];
+                    zm[u] = k != 0 ? pm[-j * rs] : zk[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kZBatch; ++u) {
+                const int k = k0 + TP * u, j = j0 + u;
+                if (k < H) {
+                    g0[TP * j] = make_float2(0.5f * (zk[u].x + zm[u].x), 0.5f * (zk[u].y - zm[u].y));
+                    if (h1) g1[TP * j] = make_float2(0.5f * (zk[u].y + zm[u].y), 0.5f * (zm[u].x - zk[u].x));
+                }
+            }
         }
     }
 }
@@ -641,12 +655,17 @@ __device__ __forceinline__ void iz_epilogue(const pvd_vol_d &v, const float2 *s,
                 d[u].x = (has_dens && o < Nz) ? v.dens[o0 + o] : 1.f;
                 d[u].y = (has_dens && o < Nz && h1) ? v.dens[o0 + Nz + o] : 1.f;
             }
+            float2 vals[kZBatch];                   // all LDS reads of the batch before any store
+#pragma unroll
+            for (int u = 0; u < kZBatch; ++u) {
+                const int j = jb + u;
+                vals[u] = oo + TP * u < Nz ? p[j * rs - (j >= jw ? rw : 0)] : make_float2(0.f, 0.f);
+            }
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
                 const int o = oo + TP * u;
                 if (o < Nz) {
-                    const int j = jb + u;
-                    float2 val = p[j * rs - (j >= jw ? rw : 0)];
+                    float2 val = vals[u];
                     if (o < az.foldn) {
                         const float2 f = s[zrow(az, (o + az.N + az.off) % Pz) * ld + q];
                         val.x += f.x;
