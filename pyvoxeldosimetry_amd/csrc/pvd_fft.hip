@@ -2,16 +2,19 @@
 //
 // The dose is computed as five descriptor-driven passes over an in-place
 // complex workspace work[Nx][Py][Hp] (H = Pz/2+1 used columns, rows padded to
-// Hp = 16*ceil(H/16) complex = whole 128-byte lines):
+// Hp = 16*ceil(H/16) complex = whole 128-byte lines; x-planes wplane apart):
 //   FZ  z: weighted sum of the T activity maps, zero-pad, two-for-one real FFT
 //   FY  y: forward FFT of every (x, kz) column
 //   X   x: forward FFT, times the kernel spectrum, inverse FFT, fold/roll map
 //   IY  y: inverse FFT, fold/roll map
 //   IZ  z: two-for-one inverse real FFT, fold/roll map, scale * rho_w/rho, store
-// Every pass stages its lines in LDS (element (line l, index k) at s[k*ld + l])
-// so global accesses are coalesced along the contiguous axis; all global loads
-// of a tile are issued before any is consumed (unrolled, predicated) so each
-// thread keeps many requests in flight.
+// Transforms: 256-point y and x axes run a register-resident 16x16 four-step
+// (one LDS exchange); z lengths P = A * B of the kZregA/kZregB table a register
+// two-step; every other length LDS Stockham stages over lines staged k-major
+// (element (line l, index k) at s[k*ld + l]) so global accesses stay coalesced
+// along the contiguous axis.  The global loads of a tile are issued before any
+// is consumed, and the barriers order LDS only (lds_barrier), so loads stay in
+// flight across them.
 #include "pvd_internal.h"
 
 namespace {
