@@ -53,7 +53,8 @@ int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const float *dens
  * padding outside the grid.  cout is always 32; cin is 16 or 32. */
 #define PVD_CNN_EPI_RELU        0  /* a = relu(z + b): writes a (bf16) to y and to act  */
 #define PVD_CNN_EPI_RESID_RELU  1  /* a = relu(z + b): act = a, y = resid + a (bf16)     */
-/* (act may be NULL in both: inference keeps no activations for a backward pass) */
+/* (act may be NULL in both: inference keeps no activations for a backward pass;
+   y and act must be 16-byte aligned: whole 32-voxel tiles are written in 16-byte chunks) */
 #define PVD_CNN_EPI_LINEAR_F32  2  /* y32[v] = z[v][0] + b[0]  (FP32, channel 0 only)    */
 #define PVD_CNN_EPI_ACCUM_F32   3  /* y32[v][c] += z[v][c]      (FP32, all 32 channels)   */
 

@@ -55,9 +55,13 @@ struct ConvArgs {
 constexpr int FWD_BX = 2, FWD_BY = 4, FWD_THREADS = 512;
 constexpr int HX = FWD_BX + 2, HY = FWD_BY + 2, HZ = 34;
 
+// per-wave staging of one 32 x 32 BF16 output tile (32 voxels x 32 channels)
+constexpr int FWD_STAGE = 32 * 32 * 2;
+
 template <int CIN>
 constexpr size_t fwd_lds_bytes() {
-    return (size_t)27 * 32 * (CIN * 2 + 16) + (size_t)HX * HY * HZ * (CIN * 2 + 16);
+    return (size_t)27 * 32 * (CIN * 2 + 16) + (size_t)HX * HY * HZ * (CIN * 2 + 16) +
+           (size_t)(FWD_THREADS / 64) * FWD_STAGE;
 }
 
 template <int CIN, int EPI>
@@ -70,6 +74,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
     char *wl = lds;
     char *hl = lds + 27 * 32 * WS;
     const int tid = threadIdx.x;
+    char *stg = hl + HX * HY * HZ * VS + (tid >> 6) * FWD_STAGE;   // this wave's output staging
     for (int i = tid; i < 27 * 32 * C8; i += FWD_THREADS) {
         const int row = i / C8, c8 = i - row * C8;
         *reinterpret_cast<uint4 *>(wl + row * WS + c8 * 16) = reinterpret_cast<const uint4 *>(a.w)[i];
@@ -147,26 +152,47 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
         const f32x16 acc = acc0 + acc1;
         const int co = m;
         const float bco = a.bias ? a.bias[co] : 0.0f;
+        if (EPI == PVD_CNN_EPI_RELU || EPI == PVD_CNN_EPI_RESID_RELU) {
+            // The wave's output tile (voxels z0..z0+31 of one column, 32 channels) is
+            // 2 KB contiguous in y: stage it in the wave's LDS slot as [row][channel],
+            // then write it as two 16-byte stores per lane (in place of 16 two-byte
+            // stores).  One wave reads only what it wrote, in program order.
+            uint16_t *sy = reinterpret_cast<uint16_t *>(stg);
+            const uint4 *src = reinterpret_cast<const uint4 *>(stg);
+            if (a.act) {                         // training: the activations first
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const long long v = vrow + mfma_row(i, lane);
-            const float zv = acc[i] + bco;
-            if (EPI == PVD_CNN_EPI_RELU) {
-                const uint16_t h = f2bf(fmaxf(zv, 0.0f));
-                a.y[v * 32 + co] = h;
-                if (a.act) a.act[v * 32 + co] = h;
-            } else if (EPI == PVD_CNN_EPI_RESID_RELU) {
-                const float av = fmaxf(zv, 0.0f);
-                if (a.act) a.act[v * 32 + co] = f2bf(av);
-                uint16_t r = rs[i];
-                if (resid_lds)   // halo voxel (wx + 1, wy + 1, z + 1), channel co
-                    r = *reinterpret_cast<const uint16_t *>(
-                        hl + (((wx + 1) * HY + (wy + 1)) * HZ + mfma_row(i, lane) + 1) * VS + co * 2);
-                a.y[v * 32 + co] = f2bf(bf2f(r) + av);
-            } else if (EPI == PVD_CNN_EPI_LINEAR_F32) {
-                if (co == 0) a.y32[v] = zv;
-            } else {
-                a.y32[v * 32 + co] += acc[i];
+                for (int i = 0; i < 16; ++i) sy[mfma_row(i, lane) * 32 + co] = f2bf(fmaxf(acc[i] + bco, 0.0f));
+                uint4 *dst = reinterpret_cast<uint4 *>(a.act + vrow * 32);
+                dst[lane] = src[lane];
+                dst[lane + 64] = src[lane + 64];
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const float av = fmaxf(acc[i] + bco, 0.0f);
+                uint16_t h;
+                if (EPI == PVD_CNN_EPI_RELU) {
+                    h = f2bf(av);
+                } else {
+                    uint16_t r = rs[i];
+                    if (resid_lds)   // halo voxel (wx + 1, wy + 1, z + 1), channel co
+                        r = *reinterpret_cast<const uint16_t *>(
+                            hl + (((wx + 1) * HY + (wy + 1)) * HZ + mfma_row(i, lane) + 1) * VS + co * 2);
+                    h = f2bf(bf2f(r) + av);
+                }
+                sy[mfma_row(i, lane) * 32 + co] = h;
+            }
+            uint4 *dst = reinterpret_cast<uint4 *>(a.y + vrow * 32);
+            dst[lane] = src[lane];
+            dst[lane + 64] = src[lane + 64];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const long long v = vrow + mfma_row(i, lane);
+                if (EPI == PVD_CNN_EPI_LINEAR_F32) {
+                    if (co == 0) a.y32[v] = acc[i] + bco;
+                } else {
+                    a.y32[v * 32 + co] += acc[i];
+                }
             }
         }
     }
@@ -481,6 +507,10 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
         case PVD_CNN_EPI_ACCUM_F32: if (!y32) return pvd_set_error(PVD_EINVAL, "null FP32 output"); break;
         default: return pvd_set_error(PVD_EINVAL, "unknown epilogue %d", epi);
     }
+    // BF16 outputs are written as 16-byte chunks of whole 32-voxel tiles
+    if ((epi == PVD_CNN_EPI_RELU || epi == PVD_CNN_EPI_RESID_RELU) &&
+        ((reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(act)) & 15))
+        return pvd_set_error(PVD_EINVAL, "y and act must be 16-byte aligned");
     ConvArgs a{(const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)resid, (uint16_t *)y,
                (uint16_t *)act, y32, n0, n1, n2};
     const long long total = (long long)((n0 + FWD_BX - 1) / FWD_BX) * ((n1 + FWD_BY - 1) / FWD_BY) * (n2 >> 5);
