@@ -84,6 +84,9 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
     const int wave = tid >> 6, lane = tid & 63;
     const int wx = wave / FWD_BY, wy = wave % FWD_BY;
     const int m = lane & 31, kh = lane >> 5;
+    // this lane's output-channel bias, loaded once: a global load in the epilogue
+    // would wait (in-order counter) for the next block's halo prefetch
+    const float bco = a.bias ? a.bias[m] : 0.0f;
     // The halo of the next block is fetched into registers while this block's
     // MFMA loop runs (all loads in flight at once), then written to LDS.
     constexpr int HALO = HX * HY * HZ * C8;
@@ -151,7 +154,6 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
         }
         const f32x16 acc = acc0 + acc1;
         const int co = m;
-        const float bco = a.bias ? a.bias[co] : 0.0f;
         if (EPI == PVD_CNN_EPI_RELU || EPI == PVD_CNN_EPI_RESID_RELU) {
             // The wave's output tile (voxels z0..z0+31 of one column, 32 channels) is
             // 2 KB contiguous in y: stage it in the wave's LDS slot as [row][channel],
