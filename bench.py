@@ -336,6 +336,24 @@ def bench_torso(args):
            "dtype": "f32", "data": "synthetic (torso phantom seed 5)",
            "config": {"workload": "torso 512x512x400 @0.98 mm, Lu-177 61^3, T=6, density, linear convolution",
                       "fft_n": list(plan.fft_n)}}
+    # per-pass durations (HIP events between the passes) and the dominant pass's roofline
+    prof = ConvPlan(ph.shape, ph.kernel_grid, mode="linear", algo="fft", profile=True, device=dev)
+    prof.set_kernel(kern)
+    acc = {}
+    reps = max(3, min(args.profile_reps, 20))
+    for i in range(reps + 2):
+        prof.dose_timeseries(acts, ph.time_points_h, density=rho, out=out)
+        if i >= 2:
+            for k, v in prof.timings().items():
+                acc[k] = acc.get(k, 0.0) + v / reps
+    prof.close()
+    dom = max(acc, key=acc.get)
+    pb = pass_bytes(plan, len(acts), True)
+    achieved = pb[dom] / (acc[dom] * 1e-3) / 1e9
+    res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                       "kernel_ms": round(acc[dom], 5), "algorithmic_bytes": pb[dom]}
+    res["per_pass_ms"] = {k: round(v, 5) for k, v in acc.items()}
     if not args.no_cpu:
         from oracle import conv
         w = conv.trapezoid_weights_seconds(ph.time_points_h)

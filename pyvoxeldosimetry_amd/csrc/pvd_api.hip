@@ -78,6 +78,8 @@ namespace {
     } while (0)
 
 constexpr size_t kFftLdsCap = 96 * 1024;   // per-workgroup LDS budget of the FFT passes
+// y/x lengths 16 * b with a register two-step instance (the kernels' PVD_XY_BS list)
+constexpr int kXyRegB[] = {20, 24, 25, 27, 28, 30, 32, 35, 36};
 
 bool is_smooth(int n) {
     if (n < 1) return false;
@@ -264,6 +266,13 @@ int ilog2(int v) {
     return r;
 }
 
+bool xy_two_step_len(int P) {
+    if (P == 256 || P % 16) return false;
+    for (int b : kXyRegB)
+        if (P == 16 * b) return true;
+    return false;
+}
+
 int fft_setup(pvd_plan *p) {
     pvd_vol_d &v = p->vol;
     const int Nx = p->N[0], Ny = p->N[1];
@@ -278,7 +287,11 @@ int fft_setup(pvd_plan *p) {
     int tky, by; size_t ldsy;
     pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
-    pick_tile(Px, p->tile[0], 0, &tkx, &bx, &ldsx);
+    // x passes of a two-step length default to 4-column tiles (560-point x: 466 us
+    // against 537 us with 16 columns on the 512x512x400 torso)
+    int tx_target = p->tile[0];
+    if (p->prm.tile[0] == 0 && !(p->prm.flags & PVD_FLAG_STOCKHAM) && xy_two_step_len(Px)) tx_target = 4;
+    pick_tile(Px, tx_target, 0, &tkx, &bx, &ldsx);
     // room for the padded exchange array of the register-resident 256-point path
     if (Py == 256 && tky == 16) ldsy += PVD_SQ256_EXTRA_LDS;
     if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
@@ -288,8 +301,19 @@ int fft_setup(pvd_plan *p) {
     // kernel) and the z two-step P = A * B for 16-line tiles whose block covers
     // 16*A threads in step 2 and 16*B items in two rounds in step 1
     const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
-    v.ax[0].reg = regs ? 1 : 0;
-    v.ax[1].reg = regs ? 1 : 0;
+    // y/x: 1 = the 256-point path may run; 2 = the two-step P = 16 * rb (rb in
+    // kXyRegB) in its own kernel instances
+    for (int a = 0; a < 2; ++a) {
+        pvd_axis_d &ax = v.ax[a];
+        ax.reg = regs ? 1 : 0;
+        ax.ra = ax.ra_log = ax.rb = 0;
+        if (regs && xy_two_step_len(ax.n)) {
+            ax.reg = 2;
+            ax.ra = 16;
+            ax.ra_log = 4;
+            ax.rb = ax.n / 16;
+        }
+    }
     pvd_axis_d &az = v.ax[2];
     az.reg = 0;
     az.ra = az.ra_log = az.rb = 0;
@@ -303,6 +327,9 @@ int fft_setup(pvd_plan *p) {
             break;
         }
     }
+    // the two-step instances run blocks of at most PVD_XY_MAX_BLOCK threads
+    if (v.ax[1].reg == 2) by = std::min(by, PVD_XY_MAX_BLOCK);
+    if (v.ax[0].reg == 2) bx = std::min(bx, PVD_XY_MAX_BLOCK);
     // rows padded to 128 bytes and to whole column tiles
     v.Hp = round_up(v.H, std::max(16, std::max(tky, tkx)));
     v.llz = ilog2(lz);

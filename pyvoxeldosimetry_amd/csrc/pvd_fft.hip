@@ -175,6 +175,7 @@ __device__ __forceinline__ void z_fft(float2 *s, int lnl, int ld, const pvd_axis
         case 6: zreg_fft<8, 30, INV>(s, tw); break;
         case 7: zreg_fft<16, 20, INV>(s, tw); break;
         case 8: zreg_fft<16, 25, INV>(s, tw); break;
+        case 9: zreg_fft<16, 27, INV>(s, tw); break;
         default:
             if (!RZ) fft_lines<INV>(s, lnl, ld, az, tw);
             break;
@@ -596,6 +597,191 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 }
 
 // ---------------------------------------------------------------------------------
+// Register two-step transforms of the y and x passes for lengths P = 16 * B other
+// than 256 (B = 20 ... 36), over a column tile in LDS (element (row k, line l) at
+// s[k * TK + l], TK = 2^lnl lines).  With n = B n1 + b and k = k1 + 16 k2:
+//   X[k1 + 16 k2] = sum_b W_B^(b k2) W_P^(b k1) sum_n1 x[B n1 + b] W_16^(n1 k1).
+// xy_nat2perm: natural order in; step 1 (item (b, l)) a DFT-16 over the rows
+// b + B n1 and the twiddles W_P^(b k1), written back to the same rows; step 2 (item
+// (k1, l)) a DFT-B over rows k1 B .. k1 B + B - 1.  X[k] is left in row
+// xy_row<B>(k) = (k mod 16) B + k / 16.
+// xy_perm2nat: the same split run backwards, from that order to natural order:
+// step 1 (item (k1, l)) a DFT-B over rows k1 B + k2, then W_P^(b k1); step 2 (item
+// (b, l)) a DFT-16 over rows k1 B + b, x[b + B n1] written to row b + B n1.
+// One barrier after each step, as in the Stockham stages (one per radix stage).
+template <int B>
+__device__ __forceinline__ int xy_row(int k) { return (k & 15) * B + (k >> 4); }
+// block bound of the two-step instances (their item loops are strided, so the host
+// caps the block at this; 256 VGPRs keep the X pass's DFT-30/35/36 from spilling)
+constexpr int kXyThreads = PVD_XY_MAX_BLOCK;
+
+template <int B, bool INV>
+__device__ __forceinline__ void xy_nat2perm(float2 *s, int lnl, const float2 *tw) {
+    const int TK = 1 << lnl;
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = s[(b + B * n1) * TK + l];
+        Dft<16, INV>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+            float2 t = tw[b * k1];
+            if (INV) t.y = -t.y;
+            v[k1] = cmul(v[k1], t);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) s[(b + B * k1) * TK + l] = v[k1];
+    }
+    lds_barrier();
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        float2 v[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = s[(k1 * B + n2) * TK + l];
+        dft_any<B, INV>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = v[k2];
+    }
+    lds_barrier();
+}
+
+template <int B, bool INV>
+__device__ __forceinline__ void xy_perm2nat(float2 *s, int lnl, const float2 *tw) {
+    const int TK = 1 << lnl;
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        float2 v[B];
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k2] = s[(k1 * B + k2) * TK + l];
+        dft_any<B, INV>(v);
+#pragma unroll
+        for (int b = 1; b < B; ++b) {
+            float2 t = tw[b * k1];
+            if (INV) t.y = -t.y;
+            v[b] = cmul(v[b], t);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) s[(k1 * B + b) * TK + l] = v[b];
+    }
+    lds_barrier();
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 v[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = s[(k1 * B + b) * TK + l];
+        Dft<16, INV>::run(v);
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = v[n1];
+    }
+    lds_barrier();
+}
+
+// store_col_tile for a tile left in xy_row<B> order: FFT index k sits in LDS row
+// xy_row<B>(k) (same output mapping: row o takes index (o + off) mod P, plus index
+// (o + rows_out + off) mod P when o < foldn).
+template <int B>
+__device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long long pitch,
+                                                    const float2 *__restrict__ s, int rows_out, int lnl, int P,
+                                                    int off, int foldn, int nc) {
+    const int lp = lnl - 1;
+    const int total = rows_out << lp;
+    const int pmask = (1 << lp) - 1;
+    const float4 *s4 = reinterpret_cast<const float4 *>(s);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int o = e >> lp, cp = e & pmask;
+        if ((cp << 1) >= nc) continue;
+        float4 val = s4[(xy_row<B>(wrap_add(o, off, P)) << lp) + cp];
+        if (o < foldn) {
+            const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
+            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
+        }
+        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
+    }
+}
+
+template <int B, bool INV>
+__device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ay = v.ax[1];
+    const int Ny = ay.N, Py = ay.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltky, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
+    const int x = tile / ntc;
+    const int c0 = (tile - x * ntc) << lnl;
+    const int nc = min(TK, H - c0);
+    float2 *g = v.work + (long long)x * v.wplane + c0;
+    load_twiddles(tw, ay);
+    load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
+    lds_barrier();
+    xy_nat2perm<B, INV>(s, lnl, tw);
+    if (!INV) store_col_tile_perm<B>(g, Hp, s, Py, lnl, Py, 0, 0, nc);
+    else store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
+}
+
+template <int B>
+__device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ax = v.ax[0];
+    const int Nx = ax.N, Px = ax.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltkx, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
+    const int u = tile / ntc;
+    const int y = x_tile_row(u, v.ax[1].n);
+    const int c0 = (tile - u * ntc) << lnl;
+    const int nc = min(TK, H - c0);
+    const long long plane = v.wplane;                 // work
+    const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
+    const long long off0 = (long long)y * Hp + c0;
+    float2 *g = v.work + off0;
+    const float2 *sp = v.spec + off0;
+    load_twiddles(tw, ax);
+    load_col_tile(s, g, plane, Px, Nx, lnl, nc);
+    lds_barrier();
+    xy_nat2perm<B, false>(s, lnl, tw);
+    // LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2
+    const int total = Px << lnl;
+    if (v.sym) {
+        const float *sr = reinterpret_cast<const float *>(v.spec);
+        const int Py = v.ax[1].n;
+        const long long pyh = Py / 2 + 1;
+        const long long row0 = (long long)min(y, Py - y) * Hp + c0;
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float sv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const int r = e >> lnl, l = e & (TK - 1);
+                const int k = r / B + 16 * (r % B);
+                sv[u] = (e < total && l < nc) ? sr[(long long)min(k, Px - k) * pyh * Hp + row0 + l] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < total) s[e] = cscale(s[e], sv[u]);
+            }
+        }
+    } else {
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float2 sv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const int r = e >> lnl, l = e & (TK - 1);
+                const int k = r / B + 16 * (r % B);
+                sv[u] = (e < total && l < nc) ? sp[k * splane + l] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                if (e < total) s[e] = cmul(s[e], sv[u]);
+            }
+        }
+    }
+    lds_barrier();
+    xy_perm2nat<B, true>(s, lnl, tw);
+    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
+}
+
+// ---------------------------------------------------------------------------------
 // IZ: tile = 2*nl consecutive output lines (x, y); inverse two-for-one real FFT.
 // IZ phases with TP threads per line pair: the half-spectrum load (both halves of
 // each complex line) and the epilogue (roll / fold, scale * rho_w/rho, store).
@@ -707,12 +893,18 @@ __device__ void pass_iz(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     iz_epilogue<TP>(v, s, az, L0, npair, nreal, ld, Nz, Pz);
 }
 
-template <int PASS, bool RZ, int TP>
+template <int PASS, bool RZ, int TP, int XB = 0>
 __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     extern __shared__ float2 smem[];
     const int axis = (PASS == PVD_PASS_X) ? 0 : (PASS == PVD_PASS_FY || PASS == PVD_PASS_IY) ? 1 : 2;
     float2 *tw = smem;
     float2 *s = smem + ((v.ax[axis].n + 7) & ~7);
+    if constexpr (XB > 0) {           // register two-step, P = 16 * XB (host-selected instance)
+        if (PASS == PVD_PASS_FY) pass_y_2s<XB, false>(v, tile, tw, s);
+        else if (PASS == PVD_PASS_IY) pass_y_2s<XB, true>(v, tile, tw, s);
+        else if (PASS == PVD_PASS_X) pass_x_2s<XB>(v, tile, tw, s);
+        return;
+    }
     if (PASS == PVD_PASS_FZ) pass_fz<RZ, TP>(v, tile, tw, s);
     else if (PASS == PVD_PASS_FY) {
         if (sq256_ok(v.ax[1], v.ltky)) pass_y_sq<false>(v, tile, tw, s);
@@ -732,8 +924,8 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
 // passes use the template arguments: RZ for a plan with a register z transform,
 // TP threads per line pair in their I/O phases (the host picks both; one kernel
 // per combination, as branches between them inflate the register allocation).
-template <int PASS, bool RZ = false, int TP = 16>
-__global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
+template <int PASS, bool RZ = false, int TP = 16, int XB = 0>
+__global__ __launch_bounds__(XB > 0 ? kXyThreads : 1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
         // Workgroups go to the 8 XCDs round-robin and the X grid is a multiple of 8:
@@ -742,7 +934,7 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
         tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
-    run_pass<PASS, RZ, TP>(v, tile);
+    run_pass<PASS, RZ, TP, XB>(v, tile);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
@@ -774,6 +966,19 @@ void launch_z(int tp, int grid, int block, size_t lds, hipStream_t st, const pvd
     else hipLaunchKernelGGL((pvd_fft_pass<PASS, RZ, 16>), dim3(grid), dim3(block), lds, st, v);
 }
 
+// the register two-step instance of a y or x pass for P = 16 * rb
+#define PVD_XY_BS(X) X(20) X(24) X(25) X(27) X(28) X(30) X(32) X(35) X(36)
+template <int PASS>
+void launch_xy(int rb, int grid, int block, size_t lds, hipStream_t st, const pvd_vol_d &v) {
+    switch (rb) {
+#define PVD_XY_CASE(B) \
+        case B: hipLaunchKernelGGL((pvd_fft_pass<PASS, false, 16, B>), dim3(grid), dim3(block), lds, st, v); break;
+        PVD_XY_BS(PVD_XY_CASE)
+#undef PVD_XY_CASE
+        default: break;
+    }
+}
+
 template <int PASS>
 void launch_z_batch(int tp, int grid, int block, size_t lds, hipStream_t st, const pvd_batch_args &a) {
     if (tp == 64) hipLaunchKernelGGL((pvd_fft_pass_batch<PASS, 64>), dim3(grid), dim3(block), lds, st, a);
@@ -791,11 +996,18 @@ hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block
             if (rz) launch_z<PVD_PASS_FZ, true>(tp, grid, block, lds, st, v);
             else launch_z<PVD_PASS_FZ, false>(tp, grid, block, lds, st, v);
             break;
-        case PVD_PASS_FY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, v); break;
-        case PVD_PASS_X:   // grid rounded up to a multiple of 8 (XCD bands; surplus blocks exit)
-            hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_X>, dim3((grid + 7) & ~7), dim3(block), lds, st, v);
+        case PVD_PASS_FY:
+            if (v.ax[1].reg == 2) launch_xy<PVD_PASS_FY>(v.ax[1].rb, grid, block, lds, st, v);
+            else hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_FY>, dim3(grid), dim3(block), lds, st, v);
             break;
-        case PVD_PASS_IY: hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, v); break;
+        case PVD_PASS_X:   // grid rounded up to a multiple of 8 (XCD bands; surplus blocks exit)
+            if (v.ax[0].reg == 2) launch_xy<PVD_PASS_X>(v.ax[0].rb, (grid + 7) & ~7, block, lds, st, v);
+            else hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_X>, dim3((grid + 7) & ~7), dim3(block), lds, st, v);
+            break;
+        case PVD_PASS_IY:
+            if (v.ax[1].reg == 2) launch_xy<PVD_PASS_IY>(v.ax[1].rb, grid, block, lds, st, v);
+            else hipLaunchKernelGGL(pvd_fft_pass<PVD_PASS_IY>, dim3(grid), dim3(block), lds, st, v);
+            break;
         default:
             if (rz) launch_z<PVD_PASS_IZ, true>(tp, grid, block, lds, st, v);
             else launch_z<PVD_PASS_IZ, false>(tp, grid, block, lds, st, v);
@@ -842,6 +1054,11 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes) {
     PVD_SETATTR_Z(PVD_PASS_FZ);
     PVD_SETATTR_Z(PVD_PASS_IZ);
 #undef PVD_SETATTR_Z
+#define PVD_SETATTR_XY(B)                                                                        \
+    PVD_SETATTR((pvd_fft_pass<PVD_PASS_FY, false, 16, B>)); PVD_SETATTR((pvd_fft_pass<PVD_PASS_X, false, 16, B>)); \
+    PVD_SETATTR((pvd_fft_pass<PVD_PASS_IY, false, 16, B>));
+    PVD_XY_BS(PVD_SETATTR_XY)
+#undef PVD_SETATTR_XY
 #undef PVD_SETATTR
     return e;
 }

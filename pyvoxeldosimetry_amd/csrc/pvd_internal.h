@@ -32,8 +32,9 @@
 //   foldn outputs o < foldn also add FFT index (o + N + off) mod n
 //         (reference-mode circular wrap when the FFT is padded, P != N)
 //   reg   register-resident transform: on y/x, 1 = the 256-point 16x16 path may
-//         run; on z, index into kZregA/kZregB (0 = LDS Stockham stages)
-//   ra, ra_log, rb  the z split n = ra * rb (ra a power of two), when reg != 0
+//         run, 2 = the two-step n = 16 * rb (its own kernel instances); on z, index
+//         into kZregA/kZregB (0 = LDS Stockham stages)
+//   ra, ra_log, rb  the split n = ra * rb (ra a power of two), when it is a two-step
 struct pvd_axis_d {
     int n;
     int N;
@@ -306,9 +307,11 @@ __device__ __forceinline__ void dft_any(float2 *v) {
 
 // z lengths with a register two-step transform n = A * B (A a power of two):
 // code c of pvd_axis_d.reg selects (kZregA[c], kZregB[c]); 0 = none.
-#define PVD_ZREG_CODES 9
-constexpr int kZregA[PVD_ZREG_CODES] = {0, 8, 8, 8, 8, 16, 8, 16, 16};
-constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25};
+#define PVD_ZREG_CODES 10
+// largest block of the y/x register two-step kernel instances
+#define PVD_XY_MAX_BLOCK 512
+constexpr int kZregA[PVD_ZREG_CODES] = {0, 8, 8, 8, 8, 16, 8, 16, 16, 16};
+constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25, 27};
 
 // ---------------------------------------------------------------------------------
 // a / d for 0 <= a < 2^22 and 1 <= d <= 2^16 from a float reciprocal plus one

@@ -339,7 +339,7 @@ def test_tile_and_radix_choices_keep_parity(tile, max_radix):
 
 # z lengths with a register two-step transform P = A * B (A = 8 or 16), each in
 # reference mode (P = Nz) and one in linear mode (P chosen by the plan)
-ZREG_LENGTHS = [128, 160, 192, 200, 240, 256, 320, 400]
+ZREG_LENGTHS = [128, 160, 192, 200, 240, 256, 320, 400, 432]
 
 
 @pytest.mark.parametrize("nz", ZREG_LENGTHS)
@@ -392,3 +392,37 @@ def test_register_z_two_step_linear_and_batch():
     outs = b.run(graph=True)
     for s, a, o in zip(shapes, acts, outs):
         assert_parity(o.cpu().numpy(), conv.linear_same(a.astype(np.float64), k), what=f"batch {s}")
+
+
+# y/x lengths P = 16 * B with a register two-step transform
+XY_TWO_STEP_B = [20, 24, 25, 27, 28, 30, 32, 35, 36]
+
+
+@pytest.mark.parametrize("b", XY_TWO_STEP_B)
+def test_xy_register_two_step_against_oracle_and_stockham(b):
+    """The y and x passes of length P = 16 * B (not 256) take the register two-step
+    (natural to permuted order and back, with the permuted stores): reference mode
+    with P = N on either axis, and padded reference mode (fft_n > N: circular fold
+    and origin offset on the output rows), symmetric and general kernels; each
+    matches the FP64 oracle and the LDS Stockham stages within FP32 rounding."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    P = 16 * b
+    rng = np.random.default_rng(b)
+    cases = [((P, 6, 10), (5, 3, 3), None, True), ((7, P, 10), (3, 5, 3), None, False),
+             ((P - 9, 5, 8), (7, 3, 3), (P, 0, 0), False), ((5, P - 5, 8), (3, 5, 3), (0, P, 0), True)]
+    for shape, kshape, fft_n, sym in cases:
+        _say("xy two-step", P, shape, kshape, fft_n)
+        a = rng.random(shape).astype(np.float32)
+        rho = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+        k = dpk.lu177_kernel(1.0, kshape) if sym else rng.random(kshape)
+        exp = conv.epilogue(conv.ref_conv(a.astype(np.float64), np.asarray(k, np.float64)), rho)
+        outs = []
+        for lds_only in (False, True):
+            plan = ConvPlan(shape, kshape, mode="reference", algo="fft", fft_n=fft_n, stockham=lds_only)
+            plan.set_kernel(k)
+            assert P in plan.fft_n[:2], plan.fft_n
+            got = plan.dose(dev_tensor(a), density=dev_tensor(rho)).cpu().numpy()
+            plan.close()
+            assert_parity(got, exp, what=f"P={P} {shape} fft_n={fft_n} stockham={lds_only}")
+            outs.append(got)
+        assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(exp).max()
