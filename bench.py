@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Gvoxel/s dose conv 256x256x200 (FP64 err≤1e-5); CNN-corrected ms/vol @ ≤2% err"
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fft_passes_ct_256x256x200.md")
+TORSO_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fft_passes_torso_512x512x400.md")
 
 
 def pass_bytes(plan, T: int, density: bool):
@@ -351,7 +352,8 @@ def bench_torso(args):
     pb = pass_bytes(plan, len(acts), True)
     achieved = pb[dom] / (acc[dom] * 1e-3) / 1e9
     res["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom, TORSO_TRAFFIC_FILE),
+                       "kernel": dom,
                        "kernel_ms": round(acc[dom], 5), "algorithmic_bytes": pb[dom]}
     res["per_pass_ms"] = {k: round(v, 5) for k, v in acc.items()}
     if not args.no_cpu:
