@@ -207,39 +207,53 @@ template <int TP>
 __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long L0, int nreal, int nl, int ld,
                                         int Nz) {
     constexpr int tpl = z_tpl<TP>();
-    // weighted sum of the T maps into rows z < Nz; pairs past the last line get zeros
+    // weighted sum of the T maps into rows z < Nz; pairs past the last line get zeros.
+    // Every load of a map is issued (at element 0 of the map where the line or index is
+    // past the end) before any is used, and masked afterwards: loads under a branch are
+    // waited for one by one.
     if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + TP j)
         const int nz4 = Nz >> 2;
         constexpr int kB4 = kZBatch / 2;
         for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
             const int q = w >> tpl, t = w & (TP - 1);
             const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
-            const long long o0 = (L0 + 2 * q) * Nz;
+            const long long l0 = ((L0 + 2 * q) * Nz) >> 2;   // float4 index of real line 2q
             for (int f0 = t; f0 < nz4; f0 += TP * kB4) {
                 float4 a[kB4], b[kB4];
+                auto load = [&](const float *map, float4 *xa, float4 *xb) {
+                    const float4 *m = reinterpret_cast<const float4 *>(map);
+#pragma unroll
+                    for (int u = 0; u < kB4; ++u) {
+                        const int f = f0 + TP * u;
+                        xa[u] = m[(f < nz4 && h0) ? l0 + f : 0];
+                        xb[u] = m[(f < nz4 && h1) ? l0 + nz4 + f : 0];
+                    }
+                };
+                load(v.act[0], a, b);
+                const float w0 = v.w[0];
 #pragma unroll
                 for (int u = 0; u < kB4; ++u) {
                     const int f = f0 + TP * u;
-                    const float4 *m = reinterpret_cast<const float4 *>(v.act[0] + o0);
-                    const float4 x = (f < nz4 && h0) ? m[f] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const float4 y = (f < nz4 && h1) ? m[nz4 + f] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    const float w0 = v.w[0];
-                    a[u] = make_float4(w0 * x.x, w0 * x.y, w0 * x.z, w0 * x.w);
-                    b[u] = make_float4(w0 * y.x, w0 * y.y, w0 * y.z, w0 * y.w);
+                    const float4 x = a[u], y = b[u];
+                    a[u] = (f < nz4 && h0) ? make_float4(w0 * x.x, w0 * x.y, w0 * x.z, w0 * x.w)
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+                    b[u] = (f < nz4 && h1) ? make_float4(w0 * y.x, w0 * y.y, w0 * y.z, w0 * y.w)
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
                 for (int tt = 1; tt < v.T; ++tt) {
-                    const float4 *m = reinterpret_cast<const float4 *>(v.act[tt] + o0);
+                    float4 xa[kB4], xb[kB4];
+                    load(v.act[tt], xa, xb);
                     const float wt = v.w[tt];
 #pragma unroll
                     for (int u = 0; u < kB4; ++u) {
                         const int f = f0 + TP * u;
                         if (f < nz4 && h0) {
-                            const float4 x = m[f];
+                            const float4 x = xa[u];
                             a[u].x = fmaf(wt, x.x, a[u].x); a[u].y = fmaf(wt, x.y, a[u].y);
                             a[u].z = fmaf(wt, x.z, a[u].z); a[u].w = fmaf(wt, x.w, a[u].w);
                         }
                         if (f < nz4 && h1) {
-                            const float4 y = m[nz4 + f];
+                            const float4 y = xb[u];
                             b[u].x = fmaf(wt, y.x, b[u].x); b[u].y = fmaf(wt, y.y, b[u].y);
                             b[u].z = fmaf(wt, y.z, b[u].z); b[u].w = fmaf(wt, y.w, b[u].w);
                         }
@@ -263,21 +277,31 @@ __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long
         const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
         const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
         for (int z0 = t; z0 < Nz; z0 += TP * kZBatch) {
+            float xa[kZBatch], xb[kZBatch];
+            auto load = [&](const float *m) {
+#pragma unroll
+                for (int u = 0; u < kZBatch; ++u) {
+                    const int z = z0 + TP * u;
+                    xa[u] = m[(z < Nz && h0) ? o0 + z : 0];
+                    xb[u] = m[(z < Nz && h1) ? o0 + Nz + z : 0];
+                }
+            };
             float2 a[kZBatch];
+            load(v.act[0]);
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
                 const int z = z0 + TP * u;
-                a[u].x = (z < Nz && h0) ? v.w[0] * v.act[0][o0 + z] : 0.f;
-                a[u].y = (z < Nz && h1) ? v.w[0] * v.act[0][o0 + Nz + z] : 0.f;
+                a[u].x = (z < Nz && h0) ? v.w[0] * xa[u] : 0.f;
+                a[u].y = (z < Nz && h1) ? v.w[0] * xb[u] : 0.f;
             }
             for (int tt = 1; tt < v.T; ++tt) {
-                const float *m = v.act[tt] + o0;
+                load(v.act[tt]);
                 const float wt = v.w[tt];
 #pragma unroll
                 for (int u = 0; u < kZBatch; ++u) {
                     const int z = z0 + TP * u;
-                    if (z < Nz && h0) a[u].x = fmaf(wt, m[z], a[u].x);
-                    if (z < Nz && h1) a[u].y = fmaf(wt, m[Nz + z], a[u].y);
+                    if (z < Nz && h0) a[u].x = fmaf(wt, xa[u], a[u].x);
+                    if (z < Nz && h1) a[u].y = fmaf(wt, xb[u], a[u].y);
                 }
             }
             float2 *d = s + z0 * ld + q;
