@@ -49,22 +49,44 @@ __global__ __launch_bounds__(BLOCK) void pvd_direct_kernel(pvd_direct_args a) {
     const int bz = oz0 - (KZ - 1) + a.sh[2];
     const bool wrap = a.wrap != 0;
     const long long plane = (long long)a.N[1] * a.N[2];
-    for (int e = threadIdx.x; e < SX * SY * SZ; e += BLOCK) {
-        const int i = e / (SY * SZ);
-        const int r = e - i * SY * SZ;
-        const int j = r / SZ;
-        const int l = r - j * SZ;
-        bool okx, oky, okz;
-        const int gx = src_index(bx + i, a.N[0], wrap, okx);
-        const int gy = src_index(by + j, a.N[1], wrap, oky);
-        const int gz = src_index(bz + l, a.N[2], wrap, okz);
-        float val = 0.f;
-        if (okx && oky && okz) {
-            const long long off = gx * plane + (long long)gy * a.N[2] + gz;
-            val = a.w[0] * a.act[0][off];
-            for (int t = 1; t < a.T; ++t) val = fmaf(a.w[t], a.act[t][off], val);
+    // halo tile: kU elements per thread per round, every load issued (at element 0 of
+    // the map where the element is outside the grid or past the tile) before any is
+    // used and masked afterwards (loads under a branch are waited for one by one)
+    constexpr int kU = 8;
+    const int total = SX * SY * SZ;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kU * BLOCK) {
+        long long off[kU];
+        bool ok[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = e0 + u * BLOCK;
+            const int i = e / (SY * SZ);
+            const int r = e - i * SY * SZ;
+            const int j = r / SZ;
+            const int l = r - j * SZ;
+            bool okx, oky, okz;
+            const int gx = src_index(bx + i, a.N[0], wrap, okx);
+            const int gy = src_index(by + j, a.N[1], wrap, oky);
+            const int gz = src_index(bz + l, a.N[2], wrap, okz);
+            ok[u] = e < total && okx && oky && okz;
+            off[u] = ok[u] ? gx * plane + (long long)gy * a.N[2] + gz : 0;
         }
-        st[e] = val;
+        float val[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) val[u] = a.act[0][off[u]];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) val[u] = ok[u] ? a.w[0] * val[u] : 0.f;
+        for (int t = 1; t < a.T; ++t) {
+            float x[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) x[u] = a.act[t][off[u]];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (ok[u]) val[u] = fmaf(a.w[t], x[u], val[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (e0 + u * BLOCK < total) st[e0 + u * BLOCK] = val[u];
     }
     for (int e = threadIdx.x; e < KX * KY * KZ; e += BLOCK) kw[e] = a.kr[e];
     __syncthreads();
@@ -93,14 +115,20 @@ __global__ __launch_bounds__(BLOCK) void pvd_direct_kernel(pvd_direct_args a) {
     const int ox = ox0 + ix, oy = oy0 + iy;
     if (ox >= a.N[0] || oy >= a.N[1]) return;
     const long long obase = ox * plane + (long long)oy * a.N[2];
+    float f[VZ];
+#pragma unroll
+    for (int i = 0; i < VZ; ++i) f[i] = a.scale;
+    if (a.dens) {           // all density loads issued (z clamped into the column) before use
+        float rho[VZ];
+#pragma unroll
+        for (int i = 0; i < VZ; ++i) rho[i] = a.dens[obase + min(oz0 + iz * VZ + i, a.N[2] - 1)];
+#pragma unroll
+        for (int i = 0; i < VZ; ++i) f[i] *= density_factor(rho[i]);
+    }
 #pragma unroll
     for (int i = 0; i < VZ; ++i) {
         const int oz = oz0 + iz * VZ + i;
-        if (oz < a.N[2]) {
-            float f = a.scale;
-            if (a.dens) f *= density_factor(a.dens[obase + oz]);
-            a.out[obase + oz] = acc[i] * f;
-        }
+        if (oz < a.N[2]) a.out[obase + oz] = acc[i] * f[i];
     }
 }
 
