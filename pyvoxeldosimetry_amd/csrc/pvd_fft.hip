@@ -585,18 +585,22 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         const int Py = v.ax[1].n;
         const long long pyh = Py / 2 + 1;
         const long long row0 = (long long)min(y, Py - y) * Hp + c0;
+        // (every load issued, at the tile's first entry where masked, before any is
+        // used: loads under a branch are waited for one by one)
         for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
             float sv[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
                 const int k = e >> lnl, l = e & (TK - 1);
-                sv[u] = (e < total && l < nc) ? sr[(long long)min(k, Px - k) * pyh * Hp + row0 + l] : 0.f;
+                const bool ok = e < total && l < nc;
+                sv[u] = sr[ok ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0];
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
-                if (e < total) s[e] = cscale(s[e], sv[u]);
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cscale(s[e], ok ? sv[u] : 0.f);
             }
         }
     } else {
@@ -606,12 +610,13 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
                 const int k = e >> lnl, l = e & (TK - 1);
-                sv[u] = (e < total && l < nc) ? sp[k * splane + l] : make_float2(0.f, 0.f);
+                sv[u] = sp[(e < total && l < nc) ? k * splane + l : 0];
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
-                if (e < total) s[e] = cmul(s[e], sv[u]);
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cmul(s[e], ok ? sv[u] : make_float2(0.f, 0.f));
             }
         }
     }
@@ -748,39 +753,43 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int Nx = ax.N, Px = ax.n, H = v.H, Hp = v.Hp;
     const int lnl = v.ltkx, TK = 1 << lnl;
     const int ntc = (H + TK - 1) >> lnl;
-    const int u = tile / ntc;
-    const int y = x_tile_row(u, v.ax[1].n);
-    const int c0 = (tile - u * ntc) << lnl;
+    const int u0 = tile / ntc;
+    const int y = x_tile_row(u0, v.ax[1].n);
+    const int c0 = (tile - u0 * ntc) << lnl;
     const int nc = min(TK, H - c0);
     const long long plane = v.wplane;                 // work
     const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
     const float2 *sp = v.spec + off0;
+    // After the forward two-step, LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2.
+    // Every spectrum load is issued at a clamped address (the tile's first entry where
+    // masked) and masked afterwards.  (Loading the first batch before the forward
+    // transform took the 35-point instance from 131 to 170 VGPRs and X from 471 to
+    // 735 us on the 512x512x400 torso: it halved the workgroups per CU.)
+    const int total = Px << lnl;
+    const float *sr = reinterpret_cast<const float *>(v.spec);
+    const long long pyh = v.ax[1].n / 2 + 1;
+    const long long row0 = (long long)min(y, v.ax[1].n - y) * Hp + c0;
+    auto sym_index = [&](int e) -> long long {
+        const int r = e >> lnl, l = e & (TK - 1);
+        const int k = r / B + 16 * (r % B);
+        return (e < total && l < nc) ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0;
+    };
     load_twiddles(tw, ax);
     load_col_tile(s, g, plane, Px, Nx, lnl, nc);
     lds_barrier();
     xy_nat2perm<B, false>(s, lnl, tw);
-    // LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2
-    const int total = Px << lnl;
     if (v.sym) {
-        const float *sr = reinterpret_cast<const float *>(v.spec);
-        const int Py = v.ax[1].n;
-        const long long pyh = Py / 2 + 1;
-        const long long row0 = (long long)min(y, Py - y) * Hp + c0;
         for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
             float sv[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * blockDim.x;
-                const int r = e >> lnl, l = e & (TK - 1);
-                const int k = r / B + 16 * (r % B);
-                sv[u] = (e < total && l < nc) ? sr[(long long)min(k, Px - k) * pyh * Hp + row0 + l] : 0.f;
-            }
+            for (int u = 0; u < kBatch; ++u) sv[u] = sr[sym_index(e0 + u * blockDim.x)];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
-                if (e < total) s[e] = cscale(s[e], sv[u]);
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cscale(s[e], ok ? sv[u] : 0.f);
             }
         }
     } else {
@@ -791,12 +800,13 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
                 const int e = e0 + u * blockDim.x;
                 const int r = e >> lnl, l = e & (TK - 1);
                 const int k = r / B + 16 * (r % B);
-                sv[u] = (e < total && l < nc) ? sp[k * splane + l] : make_float2(0.f, 0.f);
+                sv[u] = sp[(e < total && l < nc) ? k * splane + l : 0];
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const int e = e0 + u * blockDim.x;
-                if (e < total) s[e] = cmul(s[e], sv[u]);
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cmul(s[e], ok ? sv[u] : make_float2(0.f, 0.f));
             }
         }
     }
@@ -948,8 +958,8 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
 // passes use the template arguments: RZ for a plan with a register z transform,
 // TP threads per line pair in their I/O phases (the host picks both; one kernel
 // per combination, as branches between them inflate the register allocation).
-template <int PASS, bool RZ = false, int TP = 16, int XB = 0>
-__global__ __launch_bounds__(XB > 0 ? kXyThreads : 1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
+template <int PASS, bool RZ = false, int TP = 16>
+__global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
         // Workgroups go to the 8 XCDs round-robin and the X grid is a multiple of 8:
@@ -958,7 +968,24 @@ __global__ __launch_bounds__(XB > 0 ? kXyThreads : 1024) PVD_FFT_OCC void pvd_ff
         tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
-    run_pass<PASS, RZ, TP, XB>(v, tile);
+    run_pass<PASS, RZ, TP>(v, tile);
+}
+
+// The y/x register two-step for P = 16 * B: its own kernels, so that their register
+// budget does not touch the other instances.  Four waves per SIMD (at most 128 VGPRs)
+// took the 560-point FY/IY from 273 to 178 us on the 512x512x400 torso; the 28, 30
+// and 32-point X passes would spill at 128 and keep three.
+template <int B> constexpr int xy_waves() { return (B == 28 || B == 30 || B == 32) ? 3 : 4; }
+
+template <int PASS, int B>
+__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<B>())))
+void pvd_fft_pass_xy(pvd_vol_d v) {
+    int tile = blockIdx.x;
+    if (PASS == PVD_PASS_X) {       // XCD bands, as pvd_fft_pass
+        tile = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+        if (tile >= v.tiles[PVD_PASS_X]) return;
+    }
+    run_pass<PASS, false, 16, B>(v, tile);
 }
 
 // batch: grid covers the tiles of every volume; find this block's volume
@@ -996,7 +1023,7 @@ template <int PASS>
 void launch_xy(int rb, int grid, int block, size_t lds, hipStream_t st, const pvd_vol_d &v) {
     switch (rb) {
 #define PVD_XY_CASE(B) \
-        case B: hipLaunchKernelGGL((pvd_fft_pass<PASS, false, 16, B>), dim3(grid), dim3(block), lds, st, v); break;
+        case B: hipLaunchKernelGGL((pvd_fft_pass_xy<PASS, B>), dim3(grid), dim3(block), lds, st, v); break;
         PVD_XY_BS(PVD_XY_CASE)
 #undef PVD_XY_CASE
         default: break;
@@ -1079,8 +1106,8 @@ hipError_t pvd_set_fft_lds_limits(size_t bytes) {
     PVD_SETATTR_Z(PVD_PASS_IZ);
 #undef PVD_SETATTR_Z
 #define PVD_SETATTR_XY(B)                                                                        \
-    PVD_SETATTR((pvd_fft_pass<PVD_PASS_FY, false, 16, B>)); PVD_SETATTR((pvd_fft_pass<PVD_PASS_X, false, 16, B>)); \
-    PVD_SETATTR((pvd_fft_pass<PVD_PASS_IY, false, 16, B>));
+    PVD_SETATTR((pvd_fft_pass_xy<PVD_PASS_FY, B>)); PVD_SETATTR((pvd_fft_pass_xy<PVD_PASS_X, B>)); \
+    PVD_SETATTR((pvd_fft_pass_xy<PVD_PASS_IY, B>));
     PVD_XY_BS(PVD_SETATTR_XY)
 #undef PVD_SETATTR_XY
 #undef PVD_SETATTR
