@@ -287,11 +287,7 @@ int fft_setup(pvd_plan *p) {
     int tky, by; size_t ldsy;
     pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
-    // x passes of a two-step length default to 4-column tiles (560-point x: 466 us
-    // against 537 us with 16 columns on the 512x512x400 torso)
-    int tx_target = p->tile[0];
-    if (p->prm.tile[0] == 0 && !(p->prm.flags & PVD_FLAG_STOCKHAM) && xy_two_step_len(Px)) tx_target = 4;
-    pick_tile(Px, tx_target, 0, &tkx, &bx, &ldsx);
+    pick_tile(Px, p->tile[0], 0, &tkx, &bx, &ldsx);
     // room for the padded exchange array of the register-resident 256-point path
     if (Py == 256 && tky == 16) ldsy += PVD_SQ256_EXTRA_LDS;
     if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
