@@ -34,6 +34,20 @@ __device__ __forceinline__ int wrap_add(int a, int b, int n) {  // (a + b) mod n
     return r >= n ? r - n : r;
 }
 
+// Element at a 32-bit byte offset from a wave-uniform base: the load or store takes
+// the base in scalar registers and one VGPR of offset (no 64-bit address math).
+// The 256-point passes address at most Nx * wplane * 8 <= 256 * (256 * 1032 + pad) * 8
+// bytes of workspace and 129 * 129 * 1032 * 4 bytes of spectrum; an FZ tile spans at
+// most 2 * PVD_TILE_MAX lines of PVD_MAX_FFT_LEN floats: all well below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 __device__ __forceinline__ void load_twiddles(float2 *tw, const pvd_axis_d &ax) {
     for (int i = threadIdx.x; i < ax.n; i += blockDim.x) tw[i] = ax.tw[i];
 }
@@ -217,16 +231,18 @@ __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long
         for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
             const int q = w >> tpl, t = w & (TP - 1);
             const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
-            const long long l0 = ((L0 + 2 * q) * Nz) >> 2;   // float4 index of real line 2q
+            // 32-bit byte offsets from the tile's first element (wave-uniform): one VGPR
+            // per address; the tile spans at most 2 * PVD_TILE_MAX lines of Nz floats
+            const uint32_t l0 = (uint32_t)(2 * q * nz4);     // float4 index of real line 2q in the tile
             for (int f0 = t; f0 < nz4; f0 += TP * kB4) {
                 float4 a[kB4], b[kB4];
                 auto load = [&](const float *map, float4 *xa, float4 *xb) {
-                    const float4 *m = reinterpret_cast<const float4 *>(map);
+                    const float4 *m = reinterpret_cast<const float4 *>(map + L0 * Nz);
 #pragma unroll
                     for (int u = 0; u < kB4; ++u) {
                         const int f = f0 + TP * u;
-                        xa[u] = m[(f < nz4 && h0) ? l0 + f : 0];
-                        xb[u] = m[(f < nz4 && h1) ? l0 + nz4 + f : 0];
+                        xa[u] = at(m, ((f < nz4 && h0) ? l0 + f : 0u) * 16u);
+                        xb[u] = at(m, ((f < nz4 && h1) ? l0 + nz4 + f : 0u) * 16u);
                     }
                 };
                 load(v.act[0], a, b);
@@ -275,15 +291,16 @@ __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long
     } else for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
         const int q = w >> tpl, t = w & (TP - 1);
         const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
-        const long long o0 = (L0 + 2 * q) * Nz;         // element 0 of real line 2q
+        const uint32_t o0 = (uint32_t)(2 * q * Nz);     // element 0 of real line 2q in the tile
         for (int z0 = t; z0 < Nz; z0 += TP * kZBatch) {
             float xa[kZBatch], xb[kZBatch];
-            auto load = [&](const float *m) {
+            auto load = [&](const float *map) {        // byte offsets from the tile's first element
+                const float *m = map + L0 * Nz;
 #pragma unroll
                 for (int u = 0; u < kZBatch; ++u) {
                     const int z = z0 + TP * u;
-                    xa[u] = m[(z < Nz && h0) ? o0 + z : 0];
-                    xb[u] = m[(z < Nz && h1) ? o0 + Nz + z : 0];
+                    xa[u] = at(m, ((z < Nz && h0) ? o0 + z : 0u) * 4u);
+                    xb[u] = at(m, ((z < Nz && h1) ? o0 + Nz + z : 0u) * 4u);
                 }
             };
             float2 a[kZBatch];
@@ -413,19 +430,6 @@ __device__ __forceinline__ void sq_transform(float2 *v, int l, int g, const floa
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) v[n2] = s[g * kSqPlane + n2 * 16 + l];
     Dft<16, INV>::run(v);
-}
-
-// Element at a 32-bit byte offset from a wave-uniform base: the load or store takes
-// the base in scalar registers and one VGPR of offset (no 64-bit address math).
-// The 256-point passes address at most Nx * wplane * 8 <= 256 * (256 * 1032 + pad) * 8
-// bytes of workspace and 129 * 129 * 1032 * 4 bytes of spectrum, well below 4 GiB.
-template <typename T>
-__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
-}
-template <typename T>
-__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
 template <bool INV>
