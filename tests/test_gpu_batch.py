@@ -106,6 +106,33 @@ def test_batch_with_256_point_volume_matches_single_plans():
     b.close()
 
 
+def test_batch_with_two_step_length_volume():
+    """A volume whose y length is a register two-step length (560 = 16 x 35) next to
+    others: in the batch launch it takes the generic stages, alone the two-step; both
+    match the FP64 oracle and each other within FP32 rounding."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    from pyvoxeldosimetry_amd.batch import DoseBatch
+    _say("batch with a 16*B-length volume")
+    shapes = [(6, 560, 12), (40, 36, 32)]
+    rng = np.random.default_rng(45)
+    kernels = [dpk.lu177_kernel(2.0, (3, 9, 3)), dpk.y90_kernel(4.0, (11, 11, 11), fix_centre=True)]
+    acts = [rng.random(s).astype(np.float32) for s in shapes]
+    b = DoseBatch(shapes, kernels, mode="reference")
+    for i in range(len(shapes)):
+        b.set_volume(i, [dev(acts[i])], [1.0])
+    outs = [o.cpu().numpy() for o in b.run(graph=True)]
+    for i, shp in enumerate(shapes):
+        exp = conv.ref_conv(acts[i].astype(np.float64), kernels[i])
+        plan = ConvPlan(shp, kernels[i].shape, mode="reference", algo="fft")
+        plan.set_kernel(kernels[i])
+        single = plan.dose(dev(acts[i])).cpu().numpy()
+        plan.close()
+        assert conv.error_metrics(outs[i], exp)["max_rel"] <= MAX_REL, f"batch volume {i}"
+        assert conv.error_metrics(single, exp)["max_rel"] <= MAX_REL, f"single volume {i}"
+        assert np.abs(outs[i] - single).max() <= 1e-6 * np.abs(exp).max()
+    b.close()
+
+
 def test_batch_errors():
     from pyvoxeldosimetry_amd.batch import DoseBatch
     _say("batch errors")

@@ -426,3 +426,25 @@ def test_xy_register_two_step_against_oracle_and_stockham(b):
             assert_parity(got, exp, what=f"P={P} {shape} fft_n={fft_n} stockham={lds_only}")
             outs.append(got)
         assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(exp).max()
+
+
+def test_xy_two_step_linear_time_series_with_density():
+    """Linear mode (no fold, offset 0) on a 16 * B length (y: 313 + 7 -> 320 = 16 x 20)
+    with a 3-timepoint trapezoid and density: the time-integrated dose matches the FP64
+    oracle."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("xy two-step linear time series")
+    rng = np.random.default_rng(320)
+    shape, kshape = (29, 313, 24), (15, 15, 5)
+    acts = [rng.random(shape).astype(np.float32) for _ in range(3)]
+    rho = rng.uniform(0.5, 1.5, size=shape).astype(np.float32)
+    t_h = [1.0, 4.0, 24.0]
+    k = dpk.lu177_kernel(2.0, kshape)
+    plan = ConvPlan(shape, kshape, mode="linear", algo="fft")
+    plan.set_kernel(k)
+    assert plan.fft_n[1] == 320, plan.fft_n
+    got = plan.dose_timeseries([dev_tensor(a) for a in acts], t_h, density=dev_tensor(rho)).cpu().numpy()
+    plan.close()
+    w = conv.trapezoid_weights_seconds(t_h)
+    acc = sum(wi * a.astype(np.float64) for wi, a in zip(w, acts))
+    assert_parity(got, conv.epilogue(conv.linear_same(acc, k), rho), what="linear two-step time series")
