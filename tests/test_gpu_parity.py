@@ -448,3 +448,19 @@ def test_xy_two_step_linear_time_series_with_density():
     w = conv.trapezoid_weights_seconds(t_h)
     acc = sum(wi * a.astype(np.float64) for wi, a in zip(w, acts))
     assert_parity(got, conv.epilogue(conv.linear_same(acc, k), rho), what="linear two-step time series")
+
+
+@pytest.mark.parametrize("shape,kshape", [((1, 320, 2), (1, 5, 1)), ((2, 3, 400), (1, 3, 9)), ((400, 1, 3), (7, 1, 1))])
+def test_register_paths_on_thin_volumes(shape, kshape):
+    """Register two-step lengths next to 1-, 2- and 3-voxel axes (reference mode, P = N):
+    the dose matches the FP64 oracle."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("thin volume", shape, kshape)
+    rng = np.random.default_rng(sum(shape))
+    a = rng.random(shape).astype(np.float32)
+    k = rng.random(kshape)
+    plan = ConvPlan(shape, kshape, mode="reference", algo="fft")
+    plan.set_kernel(k)
+    got = plan.dose(dev_tensor(a)).cpu().numpy()
+    plan.close()
+    assert_parity(got, conv.ref_conv(a.astype(np.float64), k), what=f"thin {shape}")
