@@ -44,6 +44,13 @@ extern "C" {
 #define PVD_FLAG_PROFILE 1  /* record HIP events around every pass (pvd_plan_timings) */
 #define PVD_FLAG_GENERAL_SPECTRUM 2  /* keep the full complex spectrum even for symmetric kernels */
 #define PVD_FLAG_STOCKHAM 4  /* LDS Stockham stages on every axis (no register-resident transforms) */
+#define PVD_FLAG_SMOOTH_LENGTHS 8  /* automatic FFT lengths from the 7-smooth rule only: no padding
+                                      toward lengths with a register-resident transform */
+
+/* Transform of one FFT axis (pvd_plan_info.fft_path). */
+#define PVD_PATH_STOCKHAM 0  /* LDS Stockham radix stages                                  */
+#define PVD_PATH_SQ256    1  /* y/x, length 256: register 16 x 16 four-step, one LDS exchange */
+#define PVD_PATH_TWO_STEP 2  /* register two-step P = A * B (y/x: 16 * B; z: A in {8, 16})  */
 
 #define PVD_MAX_TIMEPOINTS 16
 #define PVD_MAX_FFT_LEN    2048
@@ -72,6 +79,9 @@ typedef struct pvd_plan_info {
     size_t workspace_bytes; /* device bytes owned by the plan                    */
     int tile[3];            /* lines per tile used by the x, y and z passes      */
     int max_radix;          /* largest power-of-two radix used                   */
+    int fft_path[3];        /* PVD_PATH_* of the x, y and z transforms (FFT)     */
+    int block[5];           /* threads per workgroup of the FZ, FY, X, IY, IZ passes */
+    int grid[5];            /* workgroups (tiles) of each pass                   */
 } pvd_plan_info;
 
 /* Dose-point-kernel terms (the analytic generators of
@@ -107,6 +117,11 @@ int pvd_dpk_generate(double voxel_mm, int gx, int gy, int gz,
  * KernelConvolutionCalculator.calculate_dose_rate (kernel_convolution.py:48-76). */
 int pvd_plan_create(pvd_plan **plan, const pvd_plan_params *params);
 int pvd_plan_info_get(const pvd_plan *plan, pvd_plan_info *info);
+/* Host only, no device call: the plan pvd_plan_create would build on a gfx950
+ * device for these parameters (FFT lengths, algorithm, transform paths, tiles,
+ * launch shapes, workspace bytes; symmetric_spectrum is 0 as no kernel is given).
+ * No counterpart in the reference, which plans nothing (kernel_convolution.py:71-74). */
+int pvd_plan_describe(const pvd_plan_params *params, pvd_plan_info *info);
 int pvd_plan_destroy(pvd_plan *plan);
 
 /* Upload the FP64 dose kernel (device pointer, k[0]*k[1]*k[2] doubles) and

@@ -26,6 +26,11 @@ PVD_ALGO_DIRECT = 2
 PVD_FLAG_PROFILE = 1
 PVD_FLAG_GENERAL_SPECTRUM = 2
 PVD_FLAG_STOCKHAM = 4
+PVD_FLAG_SMOOTH_LENGTHS = 8
+PVD_PATH_STOCKHAM = 0
+PVD_PATH_SQ256 = 1
+PVD_PATH_TWO_STEP = 2
+PATHS = {PVD_PATH_STOCKHAM: "stockham", PVD_PATH_SQ256: "sq256", PVD_PATH_TWO_STEP: "two_step"}
 PVD_MAX_TIMEPOINTS = 16
 PVD_MAX_FFT_LEN = 2048
 
@@ -46,7 +51,8 @@ class PlanParams(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("algo", ctypes.c_int), ("symmetric_spectrum", ctypes.c_int), ("fft_n", ctypes.c_int * 3),
                 ("kernel_used", ctypes.c_int * 3), ("n_passes", ctypes.c_int), ("workspace_bytes", ctypes.c_size_t),
-                ("tile", ctypes.c_int * 3), ("max_radix", ctypes.c_int)]
+                ("tile", ctypes.c_int * 3), ("max_radix", ctypes.c_int), ("fft_path", ctypes.c_int * 3),
+                ("block", ctypes.c_int * 5), ("grid", ctypes.c_int * 5)]
 
 
 class DpkTerm(ctypes.Structure):
@@ -64,6 +70,7 @@ SIGNATURES = {
     "pvd_dpk_generate": (_I, [_D, _I, _I, _I, ctypes.POINTER(DpkTerm), _I, _D, _P, _P]),
     "pvd_plan_create": (_I, [ctypes.POINTER(_P), ctypes.POINTER(PlanParams)]),
     "pvd_plan_info_get": (_I, [_P, ctypes.POINTER(PlanInfo)]),
+    "pvd_plan_describe": (_I, [ctypes.POINTER(PlanParams), ctypes.POINTER(PlanInfo)]),
     "pvd_plan_destroy": (_I, [_P]),
     "pvd_plan_set_kernel": (_I, [_P, _P, _P]),
     "pvd_conv_dose": (_I, [_P, _P, _P, _P, _D, _P]),

@@ -1,7 +1,8 @@
 """Measured choice of algorithm, FFT lengths, tile shapes and radix per (grid, kernel, mode).
 
-Candidates: the FFT pipeline at the plan's default lengths and at a few other
-7-smooth lengths that do not alias, and the direct LDS convolution when the
+Candidates: the FFT pipeline at the plan's default lengths (7-smooth, padded toward
+lengths with a register-resident transform) and at the smallest 7-smooth lengths
+that do not alias, and the direct LDS convolution when the
 kernel extent allows.  For the fastest FFT lengths the tuner then walks the
 tile size of each pass (8 / 16 / 32 lines per workgroup, z then y then x), the
 largest Stockham radix (16 or 8) and the LDS Stockham stages in place of the
@@ -15,7 +16,7 @@ import json
 import os
 import threading
 
-from .plan import ConvPlan
+from .plan import ConvPlan, describe_plan
 
 _CACHE_PATH = os.environ.get("PVD_AUTOTUNE_CACHE",
                              os.path.join(os.path.expanduser("~"), ".cache", "pyvoxeldosimetry_amd", "autotune.json"))
@@ -97,9 +98,12 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
                         stockham=bool(hit.get("stockham", False)) and fft, device=dev)
         plan.set_kernel(kernel_dev)
         return plan
-    cands = []
+    # the plan's own lengths (7-smooth, padded toward register-resident transforms)
+    # first, then the smallest 7-smooth lengths per axis
+    default = describe_plan(shape, kernel_shape, mode=mode, algo="fft")["fft_n"]
+    cands = [("fft", tuple(default))]
     per_axis = [candidate_lengths(shape[i], kernel_shape[i], mode) for i in range(3)]
-    seen = set()
+    seen = {tuple(default)}
     for i in range(max(len(a) for a in per_axis)):
         fn = tuple(a[min(i, len(a) - 1)] for a in per_axis)
         if fn not in seen:

@@ -1,9 +1,11 @@
 """Batched dose convolution of many volumes of uneven sizes.
 
-All volumes go through one launch per FFT pass (``pvd_batch_*`` in
-include/pvd.h); ``run(graph=True)`` replays the five launches from a hipGraph
-captured on first use.  Each volume keeps its own plan (workspace and kernel
-spectrum), so volumes may differ in grid size, kernel and mode.
+Each volume keeps its own plan (workspace, kernel spectrum, FFT lengths), so
+volumes may differ in grid size, kernel and mode.  Per FFT pass, the volumes
+that run the same kernel instance with the same block share one launch whose
+grid concatenates their tiles (``pvd_batch_*`` in include/pvd.h); a batched
+volume therefore gives the bits of its single-volume plan.  ``run(graph=True)``
+replays the launches from a hipGraph captured on first use.
 """
 from __future__ import annotations
 
@@ -14,19 +16,12 @@ from . import _native as nat
 from .plan import ConvPlan, _as_device_f32
 
 
-# Default lines per FFT tile (x, y, z) of batched volumes: on 48 phantoms of 96^3 to
-# 384x384x320 (T = 4) 8-line x/y tiles ran the batch in 12.37 ms against 13.22 ms with
-# the single-volume default of 16 (16/8/16: 12.99, 8/8/8: 12.45, 8/4/16: 13.64 ms).
-# Axes of FFT length 256 keep 16-line tiles: their register-resident path needs them,
-# and with it a batched volume gives the same bits as its single-volume plan.
-BATCH_TILE = (8, 8, 16)
-
-
 class DoseBatch:
     def __init__(self, shapes: Sequence[Sequence[int]], kernels: Sequence, mode: str = "reference",
-                 fft_n: Optional[Sequence] = None, device=None, tile: Optional[Sequence[int]] = None):
-        """``tile``: lines per FFT tile of the (x, y, z) passes of every volume (BATCH_TILE
-        when not given; 0 entries = the single-volume default)."""
+                 fft_n: Optional[Sequence] = None, device=None, tile: Optional[Sequence[int]] = None,
+                 smooth_lengths: bool = False):
+        """``fft_n``: per-volume FFT lengths (None entries: the plan's choice); ``tile``:
+        lines per FFT tile of the (x, y, z) passes of every volume (None: the plan default)."""
         import torch
         nat.require_device()
         if len(shapes) != len(kernels) or not shapes:
@@ -35,14 +30,8 @@ class DoseBatch:
         self.plans: List[ConvPlan] = []
         for i, (shp, k) in enumerate(zip(shapes, kernels)):
             fn = fft_n[i] if fft_n else None
-            if tile is not None:
-                plan = ConvPlan(shp, tuple(k.shape), mode=mode, algo="fft", fft_n=fn, device=device, tile=tile)
-            else:
-                plan = ConvPlan(shp, tuple(k.shape), mode=mode, algo="fft", fft_n=fn, device=device)
-                t = tuple(16 if plan.fft_n[a] == 256 else BATCH_TILE[a] for a in range(2)) + (BATCH_TILE[2],)
-                if t != (16, 16, 16):                   # the plan's own default
-                    plan.close()
-                    plan = ConvPlan(shp, tuple(k.shape), mode=mode, algo="fft", fft_n=fn, device=device, tile=t)
+            plan = ConvPlan(shp, tuple(k.shape), mode=mode, algo="fft", fft_n=fn, device=device, tile=tile,
+                            smooth_lengths=smooth_lengths)
             plan.set_kernel(k)
             self.plans.append(plan)
         self.device = self.plans[0].device

@@ -17,11 +17,7 @@
 
 #include "pvd_internal.h"
 
-// kernels (pvd_fft.hip, pvd_spectrum.hip, pvd_direct.hip)
-hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds, hipStream_t st);
-hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_batch_args &a, int grid, int block, size_t lds,
-                                     hipStream_t st, int max_lines);
-hipError_t pvd_set_fft_lds_limits(size_t bytes);
+// kernels (pvd_fft*.hip launchers are declared in pvd_internal.h; pvd_spectrum.hip, pvd_direct.hip)
 hipError_t pvd_launch_dpk(double v, const int g[3], const pvd_dpk_term *terms, int nterms, double factor,
                           double *out, hipStream_t st);
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
@@ -78,8 +74,11 @@ namespace {
     } while (0)
 
 constexpr size_t kFftLdsCap = 96 * 1024;   // per-workgroup LDS budget of the FFT passes
+constexpr size_t kDescribeLds = 160 * 1024; // LDS per workgroup assumed by pvd_plan_describe (gfx950)
 // y/x lengths 16 * b with a register two-step instance (the kernels' PVD_XY_BS list)
-constexpr int kXyRegB[] = {20, 24, 25, 27, 28, 30, 32, 35, 36};
+#define PVD_XY_LIST(B) B,
+constexpr int kXyRegB[] = {PVD_XY_BS(PVD_XY_LIST)};
+#undef PVD_XY_LIST
 
 bool is_smooth(int n) {
     if (n < 1) return false;
@@ -177,6 +176,7 @@ struct pvd_plan {
     bool has_kernel = false;
     pvd_vol_d vol{};               // descriptor template
     launch_cfg cfg[PVD_NPASS_FFT];
+    size_t work_elems = 0, spec_elems = 0;   // complex elements of the workspace and spectrum
     float2 *work = nullptr;
     float2 *spec = nullptr;
     float *kr = nullptr;           // direct path: reversed cropped kernel (FP32)
@@ -187,14 +187,24 @@ struct pvd_plan {
     bool profiled_call = false;
 };
 
+// One launch of a batch pass: members that share a kernel instance and block.
+struct batch_launch_d {
+    int pass = 0;
+    int exemplar = 0;                  // a member's volume index (selects the instance)
+    launch_cfg cfg;                    // grid = tiles of all members
+    int first = 0, nmem = 0;           // members in the base/vol pools
+};
+
 struct pvd_batch {
     std::vector<pvd_plan *> plans;
     int dev = 0;
     int nvol = 0;
     pvd_vol_d *h_tab = nullptr;        // pinned host copy of the descriptor table
     pvd_vol_d *d_tab = nullptr;        // device table read by the batch kernels
+    int *d_pool = nullptr;             // device member pools: tile bases, then volume indices
+    std::vector<batch_launch_d> launches;   // in pass order
+    int pool_n = 0;
     std::vector<char> ready;           // volume inputs set
-    launch_cfg cfg[PVD_NPASS_FFT];
     hipEvent_t uploaded = nullptr;     // last table upload (host table reusable after it)
     bool upload_pending = false;
     hipStream_t cap = nullptr;         // private stream used only for graph capture
@@ -203,9 +213,37 @@ struct pvd_batch {
 
 namespace {
 
+bool xy_two_step_len(int P) {
+    if (P == 256 || P % 16) return false;
+    for (int b : kXyRegB)
+        if (P == 16 * b) return true;
+    return false;
+}
+
+int z_reg_code(int P) {
+    for (int c = 1; c < PVD_ZREG_CODES; ++c)
+        if (kZregA[c] * kZregB[c] == P) return c;
+    return 0;
+}
+
+// A length whose transform runs register-resident on axis a (0 x, 1 y, 2 z): 256 on
+// y/x (16-line tiles and no reference-mode fold), 16 * B for the two-step instances on
+// y/x, and the z two-step table (16-line tiles).
+bool register_len(int a, int P, bool fold, int tile) {
+    if (a < 2) return (P == 256 && !fold && tile == 16) || xy_two_step_len(P);
+    return tile == 16 && z_reg_code(P) != 0;
+}
+
+// Largest padding (relative to the 7-smooth length) accepted to reach a register
+// length.  x: the padded x-planes are implicit zeros and never stored, so a longer
+// x transform costs compute only; y: the y passes and X read and write Py rows;
+// z: the half spectrum H = Pz/2 + 1 sizes every pass.
+constexpr double kRegSlack[3] = {1.0, 0.3, 0.1};
+
 int axis_setup(pvd_plan *p) {
     const pvd_plan_params &q = p->prm;
     const bool ref = q.mode == PVD_MODE_REFERENCE;
+    const bool regs = !(q.flags & (PVD_FLAG_STOCKHAM | PVD_FLAG_SMOOTH_LENGTHS));
     for (int a = 0; a < 3; ++a) {
         const int N = q.n[a], K = q.k[a];
         p->N[a] = N;
@@ -226,7 +264,19 @@ int axis_setup(pvd_plan *p) {
             need = N + std::max(mhi - c, c - mlo);
         }
         int P = q.fft_n[a];
-        if (P == 0) P = (ref && is_smooth(N)) ? N : next_smooth(need);
+        if (P == 0) {
+            P = (ref && is_smooth(N)) ? N : next_smooth(need);
+            // pad toward a register-resident length when it costs little (a padded
+            // reference-mode length folds the wrap back in)
+            // (lengths up to 128 take at most two Stockham stages and are left alone)
+            const int tile = p->tile[a];
+            if (regs && P > 128 && !register_len(a, P, ref && P != N, tile)) {
+                int best = 0;
+                for (int L = need; !best && L <= std::min(PVD_MAX_FFT_LEN, 2 * need); ++L)
+                    if (is_smooth(L) && register_len(a, L, ref && L != N, tile)) best = L;
+                if (best && best <= P * (1.0 + kRegSlack[a])) P = best;
+            }
+        }
         if (!is_smooth(P)) return fail(PVD_EINVAL, "fft_n[%d]=%d is not 7-smooth", a, P);
         if (ref ? !(P == N || P >= need) : P < need)
             return fail(PVD_EINVAL, "fft_n[%d]=%d aliases: need %s%d", a, P, ref ? "N or >= " : ">= ", need);
@@ -266,14 +316,18 @@ int ilog2(int v) {
     return r;
 }
 
-bool xy_two_step_len(int P) {
-    if (P == 256 || P % 16) return false;
-    for (int b : kXyRegB)
-        if (P == 16 * b) return true;
-    return false;
+// The register-resident 256-point X pass addresses the workspace and the kernel
+// spectrum with 32-bit byte offsets from a tile's base: the x-plane walk reaches
+// (Nx - 1) * wplane * 8 bytes, the general spectrum 255 * Py * Hp * 8 bytes (the
+// real octant less).  Larger volumes take the generic X pass (64-bit offsets).
+bool x_sq256_offsets_fit(long long Nx, long long Py, long long Hp, long long wplane) {
+    const long long lim = 0xffffffffLL - 256;
+    return (Nx - 1) * wplane * 8 <= lim && 255LL * Py * Hp * 8 <= lim;
 }
 
-int fft_setup(pvd_plan *p) {
+// Host-only layout of an FFT plan: tiles, register paths, launch shapes and buffer
+// sizes (no device call).
+int fft_layout(pvd_plan *p) {
     pvd_vol_d &v = p->vol;
     const int Nx = p->N[0], Ny = p->N[1];
     const int Px = p->P[0], Py = p->P[1], Pz = p->P[2];
@@ -331,34 +385,37 @@ int fft_setup(pvd_plan *p) {
     v.llz = ilog2(lz);
     v.ltky = ilog2(tky);
     v.ltkx = ilog2(tkx);
+    // x-planes of the workspace are Py * Hp complex plus PVD_PLANE_PAD (0: padding by one
+    // or three 128-byte lines measured the same on the CT workload)
+    v.wplane = Py * v.Hp + PVD_PLANE_PAD;
+    if (Px == 256 && !x_sq256_offsets_fit(Nx, Py, v.Hp, v.wplane) && v.ax[0].reg == 1) v.ax[0].reg = 0;
     const long long nlines = (long long)Nx * Ny;
     const long long tz = (nlines + 2 * lz - 1) / (2 * lz);
     const long long ty = (long long)Nx * ((v.H + tky - 1) / tky);
     const long long tx = (long long)Py * ((v.H + tkx - 1) / tkx);
-    if (tz > 0x7fffffffLL || ty > 0x7fffffffLL || tx > 0x7fffffffLL) return fail(PVD_ENOTSUP, "grid too large");
+    if (tz > 0x7fffffffLL || ty > 0x7fffffffLL || tx > 0x7ffffff0LL) return fail(PVD_ENOTSUP, "grid too large");
     p->cfg[PVD_PASS_FZ] = {(int)tz, bz, ldsz};
     p->cfg[PVD_PASS_IZ] = {(int)tz, bz, ldsz};
     p->cfg[PVD_PASS_FY] = {(int)ty, by, ldsy};
     p->cfg[PVD_PASS_IY] = {(int)ty, by, ldsy};
     p->cfg[PVD_PASS_X] = {(int)tx, bx, ldsx};
-    for (int i = 0; i < PVD_NPASS_FFT; ++i) {
-        v.tiles[i] = p->cfg[i].grid;
-        v.tile_base[i] = 0;
-    }
+    for (int i = 0; i < PVD_NPASS_FFT; ++i) v.tiles[i] = p->cfg[i].grid;
+    p->work_elems = (size_t)Nx * v.wplane;
+    p->spec_elems = (size_t)Px * Py * v.Hp;
+    p->bytes = sizeof(float2) * (p->work_elems + p->spec_elems);
+    return PVD_OK;
+}
+
+int fft_alloc(pvd_plan *p) {
+    pvd_vol_d &v = p->vol;
     for (int a = 0; a < 3; ++a) {
         const float2 *tw = nullptr;
         int rc = get_twiddles(p->dev, p->P[a], &tw);
         if (rc) return rc;
         v.ax[a].tw = tw;
     }
-    // x-planes of the workspace are Py * Hp complex plus PVD_PLANE_PAD (0: padding by one
-    // or three 128-byte lines measured the same on the CT workload)
-    v.wplane = Py * v.Hp + PVD_PLANE_PAD;
-    const size_t work_elems = (size_t)Nx * v.wplane;
-    const size_t spec_elems = (size_t)Px * Py * v.Hp;
-    PVD_HIP(hipMalloc(&p->work, sizeof(float2) * work_elems), "allocate FFT workspace");
-    PVD_HIP(hipMalloc(&p->spec, sizeof(float2) * spec_elems), "allocate kernel spectrum");
-    p->bytes = sizeof(float2) * (work_elems + spec_elems);
+    PVD_HIP(hipMalloc(&p->work, sizeof(float2) * p->work_elems), "allocate FFT workspace");
+    PVD_HIP(hipMalloc(&p->spec, sizeof(float2) * p->spec_elems), "allocate kernel spectrum");
     v.work = p->work;
     v.spec = p->spec;
     return PVD_OK;
@@ -441,6 +498,93 @@ int run_dose(pvd_plan *p, const float *const *acts, const float *w, int T, const
     return launch_fft(p, v, st);
 }
 
+// Kernel instance and block of one pass of a plan, as one number: volumes with
+// equal keys share a batch launch.
+long long batch_instance_key(const pvd_plan *p, int pass) {
+    const pvd_vol_d &v = p->vol;
+    long long inst;
+    if (pass == PVD_PASS_FZ || pass == PVD_PASS_IZ) inst = (v.ax[2].reg != 0 ? 1 : 0) + 2 * v.llz;
+    else {
+        const pvd_axis_d &ax = v.ax[pass == PVD_PASS_X ? 0 : 1];
+        inst = ax.reg == 2 ? ax.rb : 0;
+    }
+    return inst * 4096 + p->cfg[pass].block;
+}
+
+int check_params(const pvd_plan_params *q) {
+    for (int a = 0; a < 3; ++a) {
+        if (q->n[a] < 1) return fail(PVD_EINVAL, "activity grid extent %d on axis %d", q->n[a], a);
+        if (q->k[a] < 1) return fail(PVD_EINVAL, "kernel grid extent %d on axis %d", q->k[a], a);
+        if (q->fft_n[a] < 0) return fail(PVD_EINVAL, "negative fft_n");
+    }
+    if (q->mode != PVD_MODE_REFERENCE && q->mode != PVD_MODE_LINEAR) return fail(PVD_EINVAL, "bad mode %d", q->mode);
+    if (q->algo < PVD_ALGO_AUTO || q->algo > PVD_ALGO_DIRECT) return fail(PVD_EINVAL, "bad algo %d", q->algo);
+    for (int a = 0; a < 3; ++a) {
+        const int t = q->tile[a];
+        if (t < 0 || t > PVD_TILE_MAX || (t & (t - 1)))
+            return fail(PVD_EINVAL, "tile[%d]=%d must be 0 or a power of two <= %d", a, t, PVD_TILE_MAX);
+    }
+    if (q->max_radix != 0 && q->max_radix != 8 && q->max_radix != 16)
+        return fail(PVD_EINVAL, "max_radix must be 0, 8 or 16 (got %d)", q->max_radix);
+    return PVD_OK;
+}
+
+void plan_init(pvd_plan *p, const pvd_plan_params *q) {
+    p->prm = *q;
+    for (int a = 0; a < 3; ++a) p->tile[a] = q->tile[a] ? q->tile[a] : PVD_TILE_DEFAULT;
+    p->max_radix = q->max_radix ? q->max_radix : 16;
+}
+
+// Host-only: FFT lengths, algorithm, and (FFT) the layout and launch shapes.
+int plan_layout(pvd_plan *p, size_t max_lds) {
+    if (int rc = axis_setup(p)) return rc;
+    const bool direct_ok = p->Ku[0] <= 16 && p->Ku[1] <= 16 && p->Ku[2] <= 16 &&
+                           pvd_direct_lds_bytes(p->Ku) <= max_lds;
+    int algo = p->prm.algo;
+    if (algo == PVD_ALGO_DIRECT && !direct_ok)
+        return fail(PVD_ENOTSUP, "direct path needs kernel extents <= 16 (got %dx%dx%d)", p->Ku[0], p->Ku[1],
+                    p->Ku[2]);
+    if (algo == PVD_ALGO_AUTO)
+        algo = (direct_ok && direct_cost_model(p) < fft_cost_model(p)) ? PVD_ALGO_DIRECT : PVD_ALGO_FFT;
+    p->algo = algo;
+    if (algo == PVD_ALGO_FFT) return fft_layout(p);
+    p->bytes = sizeof(float) * (size_t)p->Ku[0] * p->Ku[1] * p->Ku[2];
+    return PVD_OK;
+}
+
+int axis_path(const pvd_plan *p, int a) {
+    const pvd_axis_d &ax = p->vol.ax[a];
+    if (a == 2) return ax.reg ? PVD_PATH_TWO_STEP : PVD_PATH_STOCKHAM;
+    if (ax.reg == 2) return PVD_PATH_TWO_STEP;
+    const int lines = 1 << (a == 0 ? p->vol.ltkx : p->vol.ltky);
+    const int block = p->cfg[a == 0 ? PVD_PASS_X : PVD_PASS_FY].block;
+    // the conditions the kernels check before taking the 256-point path (sq256_ok)
+    if (ax.reg && ax.n == 256 && lines == 16 && block == 256 && ax.foldn == 0) return PVD_PATH_SQ256;
+    return PVD_PATH_STOCKHAM;
+}
+
+void fill_info(const pvd_plan *p, pvd_plan_info *info) {
+    memset(info, 0, sizeof *info);
+    info->algo = p->algo;
+    info->symmetric_spectrum = p->vol.sym;
+    const bool fft = p->algo == PVD_ALGO_FFT;
+    for (int a = 0; a < 3; ++a) {
+        info->fft_n[a] = fft ? p->P[a] : 0;
+        info->kernel_used[a] = p->Ku[a];
+        info->fft_path[a] = fft ? axis_path(p, a) : 0;
+    }
+    info->n_passes = fft ? PVD_NPASS_FFT : 1;
+    info->workspace_bytes = p->bytes;
+    info->tile[0] = fft ? 1 << p->vol.ltkx : 0;
+    info->tile[1] = fft ? 1 << p->vol.ltky : 0;
+    info->tile[2] = fft ? 1 << p->vol.llz : 0;
+    info->max_radix = fft ? p->max_radix : 0;
+    for (int i = 0; fft && i < PVD_NPASS_FFT; ++i) {
+        info->block[i] = p->cfg[i].block;
+        info->grid[i] = p->cfg[i].grid;
+    }
+}
+
 }  // namespace
 
 // =================================================================================
@@ -466,51 +610,23 @@ int pvd_dpk_generate(double voxel_mm, int gx, int gy, int gz, const pvd_dpk_term
 int pvd_plan_create(pvd_plan **out, const pvd_plan_params *q) {
     if (!out || !q) return fail(PVD_EINVAL, "null argument");
     *out = nullptr;
-    for (int a = 0; a < 3; ++a) {
-        if (q->n[a] < 1) return fail(PVD_EINVAL, "activity grid extent %d on axis %d", q->n[a], a);
-        if (q->k[a] < 1) return fail(PVD_EINVAL, "kernel grid extent %d on axis %d", q->k[a], a);
-        if (q->fft_n[a] < 0) return fail(PVD_EINVAL, "negative fft_n");
-    }
-    if (q->mode != PVD_MODE_REFERENCE && q->mode != PVD_MODE_LINEAR) return fail(PVD_EINVAL, "bad mode %d", q->mode);
-    if (q->algo < PVD_ALGO_AUTO || q->algo > PVD_ALGO_DIRECT) return fail(PVD_EINVAL, "bad algo %d", q->algo);
-    for (int a = 0; a < 3; ++a) {
-        const int t = q->tile[a];
-        if (t < 0 || t > PVD_TILE_MAX || (t & (t - 1)))
-            return fail(PVD_EINVAL, "tile[%d]=%d must be 0 or a power of two <= %d", a, t, PVD_TILE_MAX);
-    }
-    if (q->max_radix != 0 && q->max_radix != 8 && q->max_radix != 16)
-        return fail(PVD_EINVAL, "max_radix must be 0, 8 or 16 (got %d)", q->max_radix);
+    if (int rc = check_params(q)) return rc;
     pvd_plan *p = new (std::nothrow) pvd_plan();
     if (!p) return fail(PVD_ENOMEM, "host allocation");
-    p->prm = *q;
-    for (int a = 0; a < 3; ++a) p->tile[a] = q->tile[a] ? q->tile[a] : PVD_TILE_DEFAULT;
-    p->max_radix = q->max_radix ? q->max_radix : 16;
+    plan_init(p, q);
     int rc = PVD_OK;
     do {
         if (hipGetDevice(&p->dev) != hipSuccess) { rc = fail(PVD_EHIP, "no HIP device"); break; }
         size_t max_lds = 0;
         if ((rc = ensure_kernel_attrs(p->dev, &max_lds))) break;
-        if ((rc = axis_setup(p))) break;
-        const bool direct_ok = p->Ku[0] <= 16 && p->Ku[1] <= 16 && p->Ku[2] <= 16 &&
-                               pvd_direct_lds_bytes(p->Ku) <= max_lds;
-        int algo = q->algo;
-        if (algo == PVD_ALGO_DIRECT && !direct_ok) {
-            rc = fail(PVD_ENOTSUP, "direct path needs kernel extents <= 16 (got %dx%dx%d)", p->Ku[0], p->Ku[1],
-                      p->Ku[2]);
-            break;
-        }
-        if (algo == PVD_ALGO_AUTO)
-            algo = (direct_ok && direct_cost_model(p) < fft_cost_model(p)) ? PVD_ALGO_DIRECT : PVD_ALGO_FFT;
-        p->algo = algo;
-        if (algo == PVD_ALGO_FFT) {
-            if ((rc = fft_setup(p))) break;
+        if ((rc = plan_layout(p, max_lds))) break;
+        if (p->algo == PVD_ALGO_FFT) {
+            if ((rc = fft_alloc(p))) break;
         } else {
-            const size_t nk = (size_t)p->Ku[0] * p->Ku[1] * p->Ku[2];
-            if (hipMalloc(&p->kr, sizeof(float) * nk) != hipSuccess) { rc = fail(PVD_ENOMEM, "allocate kernel"); break; }
-            p->bytes = sizeof(float) * nk;
+            if (hipMalloc(&p->kr, p->bytes) != hipSuccess) { rc = fail(PVD_ENOMEM, "allocate kernel"); break; }
         }
         if (q->flags & PVD_FLAG_PROFILE) {
-            const int nev = (algo == PVD_ALGO_FFT) ? PVD_NPASS_FFT + 1 : 2;
+            const int nev = (p->algo == PVD_ALGO_FFT) ? PVD_NPASS_FFT + 1 : 2;
             for (int i = 0; i < nev; ++i)
                 if (hipEventCreate(&p->ev[i]) != hipSuccess) { rc = fail(PVD_EHIP, "create event"); break; }
             p->n_ev = nev;
@@ -524,21 +640,21 @@ int pvd_plan_create(pvd_plan **out, const pvd_plan_params *q) {
     return PVD_OK;
 }
 
+int pvd_plan_describe(const pvd_plan_params *q, pvd_plan_info *info) {
+    if (!q || !info) return fail(PVD_EINVAL, "null argument");
+    if (int rc = check_params(q)) return rc;
+    pvd_plan *p = new (std::nothrow) pvd_plan();
+    if (!p) return fail(PVD_ENOMEM, "host allocation");
+    plan_init(p, q);
+    int rc = plan_layout(p, kDescribeLds);
+    if (!rc) fill_info(p, info);
+    delete p;
+    return rc;
+}
+
 int pvd_plan_info_get(const pvd_plan *p, pvd_plan_info *info) {
     if (!p || !info) return fail(PVD_EINVAL, "null argument");
-    info->algo = p->algo;
-    info->symmetric_spectrum = p->vol.sym;
-    for (int a = 0; a < 3; ++a) {
-        info->fft_n[a] = p->algo == PVD_ALGO_FFT ? p->P[a] : 0;
-        info->kernel_used[a] = p->Ku[a];
-    }
-    info->n_passes = p->algo == PVD_ALGO_FFT ? PVD_NPASS_FFT : 1;
-    info->workspace_bytes = p->bytes;
-    const bool fft = p->algo == PVD_ALGO_FFT;
-    info->tile[0] = fft ? 1 << p->vol.ltkx : 0;
-    info->tile[1] = fft ? 1 << p->vol.ltky : 0;
-    info->tile[2] = fft ? 1 << p->vol.llz : 0;
-    info->max_radix = fft ? p->max_radix : 0;
+    fill_info(p, info);
     return PVD_OK;
 }
 
@@ -685,6 +801,7 @@ int pvd_batch_destroy(pvd_batch *b) {
     if (b->cap) (void)hipStreamDestroy(b->cap);
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
     if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_pool) (void)hipFree(b->d_pool);
     if (b->h_tab) (void)hipHostFree(b->h_tab);
     delete b;
     return PVD_OK;
@@ -709,6 +826,42 @@ int pvd_batch_create(pvd_batch **out, pvd_plan *const *plans, int nvol) {
     b->ready.assign(nvol, 0);
     int rc = PVD_OK;
     do {
+        // Each pass launches once per group of volumes that run the same kernel instance
+        // with the same block as their own plans (so a batched volume gives the bits of
+        // its single-volume plan); a group's grid concatenates its members' tiles.
+        std::vector<int> bases, vols;
+        for (int pass = 0; pass < PVD_NPASS_FFT && !rc; ++pass) {
+            std::vector<std::vector<int>> groups;
+            std::vector<long long> keys;
+            for (int i = 0; i < nvol; ++i) {
+                const long long key = batch_instance_key(plans[i], pass);
+                size_t g = 0;
+                while (g < keys.size() && keys[g] != key) ++g;
+                if (g == keys.size()) { keys.push_back(key); groups.emplace_back(); }
+                groups[g].push_back(i);
+            }
+            for (const auto &grp : groups) {
+                batch_launch_d L;
+                L.pass = pass;
+                L.exemplar = grp[0];
+                L.first = (int)bases.size();
+                L.nmem = (int)grp.size();
+                long long total = 0;
+                for (int i : grp) {
+                    const launch_cfg &pc = plans[i]->cfg[pass];
+                    L.cfg.block = pc.block;
+                    L.cfg.lds = std::max(L.cfg.lds, pc.lds);
+                    bases.push_back((int)total);
+                    vols.push_back(i);
+                    total += pc.grid;
+                }
+                if (total > 0x7ffffff0LL) { rc = fail(PVD_ENOTSUP, "batch grid too large"); break; }
+                L.cfg.grid = (int)total;
+                b->launches.push_back(L);
+            }
+        }
+        if (rc) break;
+        b->pool_n = (int)bases.size();
         if (hipSetDevice(b->dev) != hipSuccess) { rc = fail(PVD_EHIP, "set device"); break; }
         if (hipHostMalloc((void **)&b->h_tab, sizeof(pvd_vol_d) * nvol, hipHostMallocDefault) != hipSuccess) {
             rc = fail(PVD_ENOMEM, "pinned descriptor table");
@@ -718,33 +871,21 @@ int pvd_batch_create(pvd_batch **out, pvd_plan *const *plans, int nvol) {
             rc = fail(PVD_ENOMEM, "device descriptor table");
             break;
         }
+        if (hipMalloc(&b->d_pool, sizeof(int) * 2 * b->pool_n) != hipSuccess) {
+            rc = fail(PVD_ENOMEM, "device member tables");
+            break;
+        }
+        std::vector<int> pool(bases);
+        pool.insert(pool.end(), vols.begin(), vols.end());
+        if (hipMemcpy(b->d_pool, pool.data(), sizeof(int) * pool.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(PVD_EHIP, "upload member tables");
+            break;
+        }
         if (hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming) != hipSuccess) {
             rc = fail(PVD_EHIP, "create event");
             break;
         }
-        for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
-            long long total = 0;
-            launch_cfg c;
-            for (int i = 0; i < nvol; ++i) {
-                const launch_cfg &pc = plans[i]->cfg[pass];
-                c.block = std::max(c.block, pc.block);
-                c.lds = std::max(c.lds, pc.lds);
-                total += pc.grid;
-            }
-            if (total > 0x7fffffffLL) { rc = fail(PVD_ENOTSUP, "batch grid too large"); break; }
-            c.grid = (int)total;
-            b->cfg[pass] = c;
-        }
-        if (rc) break;
-        for (int i = 0; i < nvol; ++i) {
-            b->h_tab[i] = plans[i]->vol;
-            for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
-                int base = 0;
-                for (int j = 0; j < i; ++j) base += plans[j]->cfg[pass].grid;
-                b->h_tab[i].tile_base[pass] = base;
-                b->h_tab[i].tiles[pass] = plans[i]->cfg[pass].grid;
-            }
-        }
+        for (int i = 0; i < nvol; ++i) b->h_tab[i] = plans[i]->vol;
     } while (0);
     if (rc) {
         pvd_batch_destroy(b);
@@ -771,13 +912,7 @@ int pvd_batch_set_volume(pvd_batch *b, int i, const float *const *acts, const do
         PVD_HIP(hipEventSynchronize(b->uploaded), "wait for descriptor upload");
         b->upload_pending = false;
     }
-    pvd_vol_d v;
-    fill_desc(p, acts, w, T, density, dose, scale, &v);
-    for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
-        v.tile_base[pass] = b->h_tab[i].tile_base[pass];
-        v.tiles[pass] = b->h_tab[i].tiles[pass];
-    }
-    b->h_tab[i] = v;
+    fill_desc(p, acts, w, T, density, dose, scale, &b->h_tab[i]);
     b->ready[i] = 1;
     return PVD_OK;
 }
@@ -795,14 +930,16 @@ int batch_upload(pvd_batch *b, hipStream_t st) {
 }
 
 int batch_launch(pvd_batch *b, hipStream_t st) {
-    pvd_batch_args a;
-    a.vols = b->d_tab;
-    a.nvol = b->nvol;
-    int max_lines = 1;
-    for (int i = 0; i < b->nvol; ++i) max_lines = std::max(max_lines, 1 << b->h_tab[i].llz);
-    for (int pass = 0; pass < PVD_NPASS_FFT; ++pass) {
-        const launch_cfg &c = b->cfg[pass];
-        PVD_HIP(pvd_launch_fft_pass_batch(pass, a, c.grid, c.block, c.lds, st, max_lines), "launch batch FFT pass");
+    for (const batch_launch_d &L : b->launches) {
+        pvd_batch_args a;
+        a.vols = b->d_tab;
+        a.base = b->d_pool + L.first;
+        a.vol = b->d_pool + b->pool_n + L.first;
+        a.nmem = L.nmem;
+        a.ntiles = L.cfg.grid;
+        PVD_HIP(pvd_launch_fft_pass_batch(L.pass, b->plans[L.exemplar]->vol, a, L.cfg.grid, L.cfg.block, L.cfg.lds,
+                                          st),
+                "launch batch FFT pass");
     }
     return PVD_OK;
 }

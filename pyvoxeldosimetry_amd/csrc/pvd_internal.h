@@ -71,13 +71,18 @@ struct pvd_vol_d {
     int pad2_;
     float2 *work;                     // [Nx][Py][Hp] complex workspace, x-planes wplane apart
     const float2 *spec;               // [Px][Py][Hp] kernel spectrum (1/P folded in); see sym
-    int tile_base[PVD_NPASS_FFT];     // first grid tile of this volume per pass (batch)
     int tiles[PVD_NPASS_FFT];         // tiles of this volume per pass
+    int pad3_;
 };
 
+// One batch launch of one FFT pass: the members are volumes that run the same
+// kernel instance with the same block; the launch's grid concatenates their tiles.
 struct pvd_batch_args {
-    const pvd_vol_d *vols;
-    int nvol;
+    const pvd_vol_d *vols;            // descriptor table of the whole batch (device)
+    const int *base;                  // first grid tile of each member, ascending (device)
+    const int *vol;                   // descriptor index of each member (device)
+    int nmem;                         // members of this launch
+    int ntiles;                       // tiles of this launch (before any rounding of the grid)
 };
 
 // ---------------------------------------------------------------------------------
@@ -306,12 +311,20 @@ __device__ __forceinline__ void dft_any(float2 *v) {
 }
 
 // z lengths with a register two-step transform n = A * B (A a power of two):
-// code c of pvd_axis_d.reg selects (kZregA[c], kZregB[c]); 0 = none.
-#define PVD_ZREG_CODES 10
+// code c of pvd_axis_d.reg selects (kZregA[c], kZregB[c]); 0 = none.  B stays at
+// most 30: the z kernels hold the B-point DFT of every code in one switch, so the
+// largest B sets their register allocation.
+#define PVD_ZREG_CODES 21
+constexpr int kZregA[PVD_ZREG_CODES] = {0, 8, 8, 8, 8, 16, 8, 16, 16, 16, 8, 8, 8, 8, 8, 8, 16, 16, 16, 16, 16};
+constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25, 27, 12, 14, 15, 18, 21, 27, 14, 18, 21, 24, 30};
 // largest block of the y/x register two-step kernel instances
 #define PVD_XY_MAX_BLOCK 512
-constexpr int kZregA[PVD_ZREG_CODES] = {0, 8, 8, 8, 8, 16, 8, 16, 16, 16};
-constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25, 27};
+// y/x lengths 16 * B other than 256 with a register two-step instance, grouped by
+// the source file that compiles them (pvd_fft_xy1/2/3.hip)
+#define PVD_XY_BS_1(X) X(6) X(8) X(9) X(10) X(12) X(14)
+#define PVD_XY_BS_2(X) X(15) X(18) X(20) X(21) X(24) X(25)
+#define PVD_XY_BS_3(X) X(27) X(28) X(30) X(32) X(35) X(36)
+#define PVD_XY_BS(X) PVD_XY_BS_1(X) PVD_XY_BS_2(X) PVD_XY_BS_3(X)
 
 // ---------------------------------------------------------------------------------
 // a / d for 0 <= a < 2^22 and 1 <= d <= 2^16 from a float reciprocal plus one
@@ -398,6 +411,29 @@ __device__ __forceinline__ void fft_lines(float2 *__restrict__ s, int lnl, int l
         Ns *= R;
     }
 }
+
+// ---------------------------------------------------------------------------------
+// Kernel launchers (host).  A launch runs one pass either for one volume (`v`, the
+// descriptor passed by value) or for the members of a batch launch (`a`, with `v`
+// any member's host descriptor: it selects the kernel instance).
+hipError_t pvd_launch_fft_pass(int pass, const pvd_vol_d &v, int grid, int block, size_t lds, hipStream_t st);
+hipError_t pvd_launch_fft_pass_batch(int pass, const pvd_vol_d &v, const pvd_batch_args &a, int grid, int block,
+                                     size_t lds, hipStream_t st);
+hipError_t pvd_set_fft_lds_limits(size_t bytes);
+// per source file of the z instances (pvd_fft_z.hip) and of the y/x two-step
+// instances (pvd_fft_xy1/2/3.hip); `batch` selects the table-reading instance
+hipError_t pvd_fft_launch_z(int pass, bool rz, int tp, const pvd_vol_d *v, const pvd_batch_args *a, int grid,
+                            int block, size_t lds, hipStream_t st);
+hipError_t pvd_fft_launch_xy1(int pass, int rb, const pvd_vol_d *v, const pvd_batch_args *a, int grid, int block,
+                              size_t lds, hipStream_t st);
+hipError_t pvd_fft_launch_xy2(int pass, int rb, const pvd_vol_d *v, const pvd_batch_args *a, int grid, int block,
+                              size_t lds, hipStream_t st);
+hipError_t pvd_fft_launch_xy3(int pass, int rb, const pvd_vol_d *v, const pvd_batch_args *a, int grid, int block,
+                              size_t lds, hipStream_t st);
+hipError_t pvd_fft_z_set_lds(size_t bytes);
+hipError_t pvd_fft_xy1_set_lds(size_t bytes);
+hipError_t pvd_fft_xy2_set_lds(size_t bytes);
+hipError_t pvd_fft_xy3_set_lds(size_t bytes);
 
 // Record an error message for pvd_last_error() and return `code` (pvd_api.hip).
 int pvd_set_error(int code, const char *fmt, ...);

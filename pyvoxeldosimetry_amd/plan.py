@@ -27,41 +27,73 @@ def _as_device_f32(t, shape, device, what):
     return t
 
 
+def _params(shape, kernel_shape, mode, algo, fft_n, flags, tile, max_radix):
+    if mode not in nat.MODES:
+        raise ValueError(f"mode must be one of {list(nat.MODES)}, got {mode!r}")
+    if algo not in nat.ALGOS:
+        raise ValueError(f"algo must be one of {list(nat.ALGOS)}, got {algo!r}")
+    if len(shape) != 3 or len(kernel_shape) != 3:
+        raise ValueError("shape and kernel_shape must have 3 entries")
+    prm = nat.PlanParams()
+    prm.n[:] = tuple(int(s) for s in shape)
+    prm.k[:] = tuple(int(s) for s in kernel_shape)
+    prm.mode = nat.MODES[mode]
+    prm.algo = nat.ALGOS[algo]
+    prm.fft_n[:] = tuple(int(x) for x in fft_n) if fft_n else (0, 0, 0)
+    prm.flags = int(flags)
+    prm.tile[:] = tuple(int(t) for t in tile) if tile else (0, 0, 0)
+    prm.max_radix = int(max_radix)
+    return prm
+
+
+def _info_dict(info) -> dict:
+    fft = info.algo == nat.PVD_ALGO_FFT
+    return {"algo": {nat.PVD_ALGO_FFT: "fft", nat.PVD_ALGO_DIRECT: "direct"}[info.algo],
+            "fft_n": tuple(info.fft_n), "kernel_used": tuple(info.kernel_used), "n_passes": info.n_passes,
+            "workspace_bytes": int(info.workspace_bytes), "tile": tuple(info.tile),
+            "max_radix": int(info.max_radix),
+            "fft_path": tuple(nat.PATHS[p] for p in info.fft_path) if fft else None,
+            "block": tuple(info.block), "grid": tuple(info.grid)}
+
+
+def describe_plan(shape: Sequence[int], kernel_shape: Sequence[int], mode: str = "reference", algo: str = "auto",
+                  fft_n: Optional[Sequence[int]] = None, tile: Optional[Sequence[int]] = None, max_radix: int = 0,
+                  stockham: bool = False, smooth_lengths: bool = False) -> dict:
+    """The plan ``ConvPlan`` would build for these arguments on a gfx950 device (FFT
+    lengths, algorithm, transform path per axis, tiles, launch shapes), computed on the
+    host by ``pvd_plan_describe`` without touching a device."""
+    lib = nat.load()
+    flags = (nat.PVD_FLAG_STOCKHAM if stockham else 0) | (nat.PVD_FLAG_SMOOTH_LENGTHS if smooth_lengths else 0)
+    prm = _params(shape, kernel_shape, mode, algo, fft_n, flags, tile, max_radix)
+    info = nat.PlanInfo()
+    nat.check(lib.pvd_plan_describe(ctypes.byref(prm), ctypes.byref(info)), "pvd_plan_describe")
+    return _info_dict(info)
+
+
 class ConvPlan:
     """Plan for dose = scale * (rho_w/rho) * conv(activity, kernel)."""
 
     def __init__(self, shape: Sequence[int], kernel_shape: Sequence[int], mode: str = "reference",
                  algo: str = "auto", fft_n: Optional[Sequence[int]] = None, profile: bool = False,
                  device=None, general_spectrum: bool = False, tile: Optional[Sequence[int]] = None,
-                 max_radix: int = 0, stockham: bool = False):
+                 max_radix: int = 0, stockham: bool = False, smooth_lengths: bool = False):
         """``tile``: lines per FFT tile of the (x, y, z) passes (powers of two, 0 = 16);
         ``max_radix``: largest power-of-two Stockham radix (16 or 8, 0 = 16);
         ``stockham``: LDS Stockham stages on every axis instead of the register-resident
-        transforms (256-point y/x, two-step z)."""
+        transforms (256-point y/x, two-step y/x/z);
+        ``smooth_lengths``: automatic FFT lengths from the 7-smooth rule only (by default a
+        length is padded further when that reaches a register-resident transform)."""
         import torch
         nat.require_device()
         self.lib = nat.load()
-        if mode not in nat.MODES:
-            raise ValueError(f"mode must be one of {list(nat.MODES)}, got {mode!r}")
-        if algo not in nat.ALGOS:
-            raise ValueError(f"algo must be one of {list(nat.ALGOS)}, got {algo!r}")
-        if len(shape) != 3 or len(kernel_shape) != 3:
-            raise ValueError("shape and kernel_shape must have 3 entries")
+        flags = ((nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum else 0)
+                 | (nat.PVD_FLAG_STOCKHAM if stockham else 0) | (nat.PVD_FLAG_SMOOTH_LENGTHS if smooth_lengths else 0))
+        prm = _params(shape, kernel_shape, mode, algo, fft_n, flags, tile, max_radix)
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.shape = tuple(int(s) for s in shape)
         self.kernel_shape = tuple(int(s) for s in kernel_shape)
         self.mode = mode
-        prm = nat.PlanParams()
-        prm.n[:] = self.shape
-        prm.k[:] = self.kernel_shape
-        prm.mode = nat.MODES[mode]
-        prm.algo = nat.ALGOS[algo]
-        prm.fft_n[:] = tuple(int(x) for x in fft_n) if fft_n else (0, 0, 0)
-        prm.flags = ((nat.PVD_FLAG_PROFILE if profile else 0) | (nat.PVD_FLAG_GENERAL_SPECTRUM if general_spectrum
-                                                                  else 0) | (nat.PVD_FLAG_STOCKHAM if stockham else 0))
         self.stockham = bool(stockham)
-        prm.tile[:] = tuple(int(t) for t in tile) if tile else (0, 0, 0)
-        prm.max_radix = int(max_radix)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             nat.check(self.lib.pvd_plan_create(ctypes.byref(h), ctypes.byref(prm)), "pvd_plan_create")
@@ -75,6 +107,7 @@ class ConvPlan:
         self.workspace_bytes = int(info.workspace_bytes)
         self.tile = tuple(info.tile)
         self.max_radix = int(info.max_radix)
+        self.fft_path = _info_dict(info)["fft_path"]
         self.symmetric_spectrum = False
         self._kernel_dev = None
 
@@ -82,7 +115,8 @@ class ConvPlan:
     def info(self) -> dict:
         return {"algo": self.algo, "mode": self.mode, "fft_n": self.fft_n, "kernel_used": self.kernel_used,
                 "n_passes": self.n_passes, "workspace_bytes": self.workspace_bytes,
-                "symmetric_spectrum": self.symmetric_spectrum, "tile": self.tile, "max_radix": self.max_radix}
+                "symmetric_spectrum": self.symmetric_spectrum, "tile": self.tile, "max_radix": self.max_radix,
+                "fft_path": self.fft_path}
 
     def set_kernel(self, kernel) -> None:
         """FP64 dose kernel (torch tensor on the device or anything numpy accepts)."""

@@ -42,7 +42,8 @@ def test_struct_layouts_match_header():
     from pyvoxeldosimetry_amd import _native as nat
     assert ctypes.sizeof(nat.PlanParams) == 4 * 16
     assert ctypes.sizeof(nat.DpkTerm) == 24
-    assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8 + 4 * 4    # 9 ints, padding, size_t, 4 ints
+    # 9 ints, padding, size_t, 17 ints (tile, max_radix, fft_path, block, grid), tail padding
+    assert ctypes.sizeof(nat.PlanInfo) == 9 * 4 + 4 + 8 + 17 * 4 + 4
     # pvd_tissue_params: 2 ints + 2x8 + 2x32 floats + int + float, then two doubles
     assert ctypes.sizeof(nat.TissueParams) == 4 * (2 + 16 + 64 + 2) + 16
     assert nat.TissueParams.sigma.offset == 4 * (2 + 16 + 64 + 2)
@@ -184,7 +185,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     fields = {"pvd_plan_params": (nat.PlanParams, ["flags", "tile", "max_radix"]),
-              "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes", "tile", "max_radix"]),
+              "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes", "tile", "max_radix", "fft_path", "block", "grid"]),
               "pvd_dpk_term": (nat.DpkTerm, ["param"]), "pvd_mc_params": (nat.McParams, ["active", "n_active"]),
               "pvd_tissue_params": (nat.TissueParams, ["sigma", "truncate"]),
               "pvd_tac_params": (nat.TacParams, ["tol", "fail_params", "t_int"])}
