@@ -12,7 +12,8 @@ Loss of the multiplicative correction ``D = Dp (1 + r)`` against ``Dt`` on
 voxel weights ``m``: ``L = sum m (Dp r - (Dt - Dp))^2 / sum m``.
 
 Pinned by tests/test_oracle_cnn.py: ``conv3`` against ``scipy.ndimage.correlate``
-per channel pair; ``backward`` against central finite differences of ``forward``.
+per channel pair; ``backward`` against central finite differences of ``forward``;
+``forward_stored`` with an identity store against ``forward``.
 """
 from __future__ import annotations
 
@@ -85,6 +86,17 @@ def forward(params, x):
     hs.append(h)
     r = conv3(h, params[5][0])[..., 0] + params[5][1][0]
     return r, (hs, acts)
+
+
+def forward_stored(params, x, store=bf16_round):
+    """``forward`` with every activation the GPU stores between layers passed through
+    ``store`` (BF16 rounding by default): h0 = store(relu(z0)), h_l = store(h_{l-1} +
+    relu(z_l)); the sums themselves are FP64.  With ``store`` the identity it is
+    ``forward`` (pinned in tests/test_oracle_cnn.py)."""
+    h = store(np.maximum(conv3(x, params[0][0]) + params[0][1], 0.0))
+    for l in range(1, 5):
+        h = store(h + np.maximum(conv3(h, params[l][0]) + params[l][1], 0.0))
+    return conv3(h, params[5][0])[..., 0] + params[5][1][0]
 
 
 def loss_and_grad_r(r, dp, dt, m):

@@ -12,6 +12,7 @@ zero-padded at the high end (or cropped to ``[0:N]``) and therefore anchored
 at its array origin, not at its centre.
 
 Pinned by (tests/test_oracle.py):
+  * ``ref_conv_threaded`` (scipy.fft, threaded) == ``ref_conv`` to FP64 rounding;
   * ``ref_conv`` == ``brute_circular`` (pure Python loops, tiny grids) ==
     ``oracle.brute`` (independent plain C) on seeded inputs, including kernels
     larger than the grid (crop) and even/odd/anisotropic shapes;
@@ -27,6 +28,7 @@ import numpy as np
 
 __all__ = [
     "ref_conv",
+    "ref_conv_threaded",
     "brute_circular",
     "brute_linear_same",
     "circular_same",
@@ -59,6 +61,17 @@ def ref_conv(activity: np.ndarray, kernel: np.ndarray) -> np.ndarray:
     a = np.asarray(activity)
     # axes spelled out only to silence NumPy 2's deprecation; identical for 3-D input
     return np.fft.ifftn(np.fft.fftn(a) * np.fft.fftn(kernel, a.shape, axes=(0, 1, 2))).real
+
+
+def ref_conv_threaded(activity: np.ndarray, kernel: np.ndarray, workers: int = -1) -> np.ndarray:
+    """``ref_conv`` with scipy.fft's threaded FP64 transforms, for volumes where the
+    single-threaded expression takes too long in a test.  Same operator (circular,
+    kernel zero-padded at the high end or cropped); pinned against ``ref_conv``."""
+    import scipy.fft as sfft
+    a = np.asarray(activity, dtype=np.float64)
+    kp = pad_or_crop(np.asarray(kernel, dtype=np.float64), a.shape)
+    spec = sfft.rfftn(a, workers=workers) * sfft.rfftn(kp, workers=workers)
+    return sfft.irfftn(spec, s=a.shape, workers=workers)
 
 
 def brute_circular(activity: np.ndarray, kernel: np.ndarray) -> np.ndarray:

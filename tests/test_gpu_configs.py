@@ -1,0 +1,150 @@
+"""GPU tests at the sizes of the benchmark configurations that no other test runs.
+
+Batched uneven workload (48 seeded phantoms, sizes U{96..384}^2 x U{96..320}, T = 4,
+mixed Y-90 11^3 / Lu-177 31^3, reference convolution): every volume of the batch
+bit-exact against its own single-volume plan, in eager launches and in the graph,
+and six named members (both nuclides, the largest volume) against the FP64
+oracle of kernel_convolution.py:78-106 (the trapezoid over per-timepoint dose
+rates, taken by linearity as one convolution of the trapezoid-weighted activity).
+
+Learned correction at 128^3: the residual CNN's forward on the GPU against the
+FP64 network oracle on windows (six 3^3 layers see 6 voxels, so a 20^3 input
+window pins its central 8^3 outputs, and a window at the volume edge also the
+outputs next to the edge), then the corrected dose with its fallback rule.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import cnn as ocnn
+from oracle import conv
+
+pytestmark = pytest.mark.gpu
+
+MAX_REL = 1e-5
+
+
+def _say(*a):
+    print("[gpu-case]", *a, file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------------- batched workload
+CHECKED = [23, 5, 0, 4, 14, 37]     # largest (376x276x315 Y-90), largest Lu-177, and four more
+
+
+@pytest.fixture(scope="module")
+def batch48():
+    import torch
+    from pyvoxeldosimetry_amd import Lu177KernelGenerator, Y90KernelGenerator
+    from pyvoxeldosimetry_amd.batch import DoseBatch
+    from pyvoxeldosimetry_amd.phantoms import BATCH_TIMEPOINTS_H, batch_phantom, batch_specs
+    specs = batch_specs(seed=3)
+    assert len(specs) == 48
+    kern = {"Y90": Y90KernelGenerator("water", fix_centre=True).generate_kernel_device(2.0, (11, 11, 11)),
+            "Lu177": Lu177KernelGenerator("water").generate_kernel_device(2.0, (31, 31, 31))}
+    _say("batch48: plans")
+    b = DoseBatch([s.shape for s in specs], [kern[s.nuclide] for s in specs], mode="reference")
+    acts = []
+    for i, s in enumerate(specs):
+        ph = batch_phantom(s)
+        acts.append([torch.from_numpy(a).cuda() for a in ph.activity])
+        b.set_volume_timeseries(i, acts[i], BATCH_TIMEPOINTS_H)
+        if i % 12 == 11:
+            _say(f"batch48: {i + 1} volumes uploaded")
+    yield specs, kern, b, acts
+    b.close()
+
+
+def test_batch48_every_volume_bit_exact_with_its_single_plan(batch48):
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan
+    from pyvoxeldosimetry_amd.phantoms import BATCH_TIMEPOINTS_H
+    specs, kern, b, acts = batch48
+    eager = [o.clone() for o in b.run(graph=False)]
+    graph = [o.clone() for o in b.run(graph=True)]
+    torch.cuda.synchronize()
+    for i, s in enumerate(specs):
+        k = kern[s.nuclide]
+        plan = ConvPlan(s.shape, tuple(k.shape), mode="reference", algo="fft")
+        plan.set_kernel(k)
+        single = plan.dose_timeseries(acts[i], BATCH_TIMEPOINTS_H)
+        torch.cuda.synchronize()
+        assert torch.equal(eager[i], single), f"volume {i} {s.shape} eager, fft_n {plan.fft_n}"
+        assert torch.equal(graph[i], single), f"volume {i} {s.shape} graph, fft_n {plan.fft_n}"
+        plan.close()
+        if i % 12 == 11:
+            _say(f"batch48: {i + 1} volumes bit-exact")
+
+
+def test_batch48_named_members_against_the_fp64_oracle(batch48):
+    import torch
+    from pyvoxeldosimetry_amd.phantoms import BATCH_TIMEPOINTS_H, batch_phantom
+    specs, kern, b, _ = batch48
+    outs = b.run(graph=True)
+    torch.cuda.synchronize()
+    nuclides = {specs[i].nuclide for i in CHECKED}
+    assert nuclides == {"Y90", "Lu177"}
+    assert max(CHECKED, key=lambda i: np.prod(specs[i].shape)) == max(range(48), key=lambda i: np.prod(specs[i].shape))
+    w = conv.trapezoid_weights_seconds(BATCH_TIMEPOINTS_H)
+    for i in CHECKED:
+        s = specs[i]
+        ph = batch_phantom(s)
+        acc = sum(wi * a.astype(np.float64) for wi, a in zip(w, ph.activity))
+        exp = conv.ref_conv_threaded(acc, kern[s.nuclide].cpu().numpy(), workers=16)
+        m = conv.error_metrics(outs[i].cpu().numpy(), exp)
+        _say(f"batch48 member {i} {s.shape} {s.nuclide}: max_rel {m['max_rel']:.2e} rms_rel {m['rms_rel']:.2e}")
+        assert m["max_rel"] <= MAX_REL, (i, s.shape, m)
+
+
+# --------------------------------------------------------------------- learned correction, 128^3
+@pytest.fixture(scope="module")
+def cnn128():
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, synth_phantom
+    corr = LearnedCorrector.pretrained("Y90", 2.0)
+    act, rho = synth_phantom(20001, (128, 128, 128), corr.device)
+    return corr, act, rho
+
+
+def _valid(start, n=128, w=20, rf=6):
+    """Outputs of a window whose 6-voxel neighbourhood lies in the window or outside the volume."""
+    lo = 0 if start == 0 else rf
+    hi = w if start + w == n else w - rf
+    return lo, hi
+
+
+@pytest.mark.parametrize("start", [(54, 60, 57), (0, 0, 0), (108, 108, 108), (0, 64, 108)])
+def test_cnn_forward_128cubed_matches_the_network_oracle_on_windows(cnn128, start):
+    corr, act, rho = cnn128
+    shape = (128, 128, 128)
+    _say(f"cnn 128^3 window {start}")
+    dp = corr.plain(act, rho)
+    x, _ = corr.pack(act, rho, dp)
+    r = corr.net.forward(x, shape, keep_activations=False).reshape(shape).cpu().numpy().astype(np.float64)
+    xh = x.float().cpu().numpy().astype(np.float64).reshape(shape + (16,))
+    params = corr.net.params_numpy()
+    sl = tuple(slice(s, s + 20) for s in start)
+    ro = ocnn.forward_stored(params, xh[sl])
+    v = [_valid(s) for s in start]
+    got = r[tuple(slice(s + lo, s + hi) for s, (lo, hi) in zip(start, v))]
+    exp = ro[tuple(slice(lo, hi) for lo, hi in v)]
+    scale = max(np.abs(exp).max(), 1e-6)
+    diff = np.abs(got - exp).max() / scale
+    fro = np.linalg.norm(got - exp) / max(np.linalg.norm(exp), 1e-30)
+    _say(f"window {start}: |r| max {scale:.3e}, max diff {diff:.2e} of max, rel fro {fro:.2e}")
+    assert np.abs(exp).max() > 1e-4                    # the trained network corrects here
+    assert diff <= 2e-2 and fro <= 5e-3
+
+
+def test_corrected_dose_128cubed_applies_the_bounded_correction(cnn128):
+    import torch
+    corr, act, rho = cnn128
+    _say("cnn 128^3 corrected dose")
+    c = corr.correct(act, rho)
+    assert not c.fallback
+    assert torch.isfinite(c.r).all() and torch.isfinite(c.dose).all()
+    ok = c.r.abs() <= corr.bound
+    expected = torch.where(ok, c.plain * (1.0 + c.r), c.plain)
+    torch.testing.assert_close(c.dose, expected, rtol=2e-6, atol=0)
+    region = c.plain >= 0.1 * c.plain.max()
+    assert c.exceed == int((~ok & region).sum().item())

@@ -339,7 +339,7 @@ def test_tile_and_radix_choices_keep_parity(tile, max_radix):
 
 # z lengths with a register two-step transform P = A * B (A = 8 or 16), each in
 # reference mode (P = Nz) and one in linear mode (P chosen by the plan)
-ZREG_LENGTHS = [128, 160, 192, 200, 240, 256, 320, 400, 432]
+ZREG_LENGTHS = [96, 112, 120, 128, 144, 160, 168, 192, 200, 216, 224, 240, 256, 288, 320, 336, 384, 400, 432, 480]
 
 
 @pytest.mark.parametrize("nz", ZREG_LENGTHS)
@@ -368,25 +368,30 @@ def test_register_z_two_step_against_oracle_and_stockham(nz):
         assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(exp).max()
 
 
-def test_register_z_two_step_linear_and_batch():
-    """Linear mode picks padded z lengths of the register table (160, 200; 210 is not), and a
-    batch of volumes with different z lengths runs each with its own transform."""
+@pytest.mark.parametrize("smooth_lengths", [False, True])
+def test_register_z_two_step_linear_and_batch(smooth_lengths):
+    """Linear mode picks padded z lengths: 160 and 200 are register lengths; for 200 + 7
+    the 7-smooth rule gives 210 (LDS Stockham) and the default pads to 216 = 8 x 27 (a
+    register length).  A batch of volumes with different z lengths runs each with its
+    own transform."""
     from pyvoxeldosimetry_amd import ConvPlan
     from pyvoxeldosimetry_amd.batch import DoseBatch
     rng = np.random.default_rng(77)
     k = dpk.lu177_kernel(2.0, (15, 15, 15))
     shapes = [(20, 18, 146), (16, 24, 190), (9, 11, 200)]      # linear "same": P >= Nz + 7
+    expect_z = [160, 200, 210 if smooth_lengths else 216]
     acts = [rng.random(s).astype(np.float32) for s in shapes]
-    for s, a in zip(shapes, acts):
-        _say("register z linear", s)
-        plan = ConvPlan(s, k.shape, mode="linear", algo="fft")
+    for s, a, pz in zip(shapes, acts, expect_z):
+        _say("register z linear", s, smooth_lengths)
+        plan = ConvPlan(s, k.shape, mode="linear", algo="fft", smooth_lengths=smooth_lengths)
         plan.set_kernel(k)
-        assert plan.fft_n[2] in (160, 200, 210), plan.fft_n
+        assert plan.fft_n[2] == pz, plan.fft_n
+        assert (plan.fft_path[2] == "two_step") == (pz != 210)
         assert_parity(plan.dose(dev_tensor(a)).cpu().numpy(), conv.linear_same(a.astype(np.float64), k),
                       what=f"linear {s} fft_n={plan.fft_n}")
         plan.close()
     _say("register z batch")
-    b = DoseBatch(shapes, [k] * len(shapes), mode="linear")
+    b = DoseBatch(shapes, [k] * len(shapes), mode="linear", smooth_lengths=smooth_lengths)
     for i, a in enumerate(acts):
         b.set_volume(i, [dev_tensor(a)], [1.0])
     outs = b.run(graph=True)
@@ -395,7 +400,7 @@ def test_register_z_two_step_linear_and_batch():
 
 
 # y/x lengths P = 16 * B with a register two-step transform
-XY_TWO_STEP_B = [20, 24, 25, 27, 28, 30, 32, 35, 36]
+XY_TWO_STEP_B = [6, 8, 9, 10, 12, 14, 15, 18, 20, 21, 24, 25, 27, 28, 30, 32, 35, 36]
 
 
 @pytest.mark.parametrize("b", XY_TWO_STEP_B)

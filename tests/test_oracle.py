@@ -205,3 +205,12 @@ def test_golden_fixtures_reproduce():
     assert set(fresh) == set(stored)
     for name, case in fresh.items():
         np.testing.assert_array_equal(np.asarray(case["expect"]), stored[name]["expect"], err_msg=name)
+
+
+def test_threaded_ref_conv_equals_the_verbatim_expression():
+    rng = np.random.default_rng(31)
+    for shape, kshape in (((24, 20, 18), (7, 7, 7)), ((9, 30, 16), (11, 3, 5)), ((6, 5, 4), (9, 9, 9))):
+        a = rng.random(shape)
+        k = rng.random(kshape)
+        np.testing.assert_allclose(conv.ref_conv_threaded(a, k, workers=2), conv.ref_conv(a, k), rtol=0,
+                                   atol=1e-12 * np.abs(conv.ref_conv(a, k)).max())

@@ -73,3 +73,16 @@ def test_bf16_round_ties_to_even():
     vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.140625], dtype=np.float32)
     got = cnn.bf16_round(vals)
     np.testing.assert_array_equal(got, np.array([1.0, 1.0, 1.0 + 2 ** -6, -2.5, 3.140625], dtype=np.float32))
+
+
+def test_forward_stored_with_identity_store_is_forward():
+    rng = np.random.default_rng(12)
+    params = cnn.init_params(rng)
+    params = [(w, rng.normal(scale=0.05, size=32)) for w, _ in params]
+    x = rng.normal(size=(5, 4, 6, 16))
+    x[..., 3:] = 0.0
+    r, _ = cnn.forward(params, x)
+    np.testing.assert_allclose(cnn.forward_stored(params, x, store=lambda a: a), r, rtol=1e-12, atol=1e-12)
+    # BF16 storage changes the result by about BF16 rounding, not more
+    rb = cnn.forward_stored(params, x)
+    assert 0 < np.abs(rb - r).max() <= 3e-2 * np.abs(r).max()
