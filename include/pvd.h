@@ -173,6 +173,20 @@ int pvd_batch_run(pvd_batch *batch, void *stream);
 int pvd_batch_run_graph(pvd_batch *batch, void *stream);
 int pvd_batch_destroy(pvd_batch *batch);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the direct-path dose of pvd_plan_create(algo = PVD_ALGO_DIRECT) +
+ * pvd_plan_set_kernel + pvd_conv_dose in one stream-ordered call, with no plan and
+ * no allocation.  n: grid (x, y, z); k: kernel grid; kernel: k[0]*k[1]*k[2] FP64 on
+ * the device; mode: PVD_MODE_*; dose = scale * (rho_w/rho) * conv(act, kernel) as
+ * pvd_conv_dose.  The kernel extent used (after the reference-mode crop, or the
+ * taps a linear-mode output reaches) must be <= 16 per axis: at most
+ * PVD_NOALLOC_MAX_TAPS taps, held in a static device array of that many floats per
+ * GPU.  Every call on a GPU shares that array: calls that overlap on one GPU race. */
+#define PVD_NOALLOC_MAX_TAPS 4096
+int pvd_conv_dose_direct_noalloc(const int n[3], const int k[3], int mode, const double *kernel,
+                                 const float *act, const float *density, float *dose, double scale, void *stream);
+
 /* Per-pass GPU milliseconds of the last dose call of a PVD_FLAG_PROFILE plan
  * (synchronises on it).  Writes min(n, n_passes) values; returns n_passes. */
 int pvd_plan_timings(pvd_plan *plan, float *ms, int n);

@@ -33,6 +33,7 @@ PVD_PATH_TWO_STEP = 2
 PATHS = {PVD_PATH_STOCKHAM: "stockham", PVD_PATH_SQ256: "sq256", PVD_PATH_TWO_STEP: "two_step"}
 PVD_MAX_TIMEPOINTS = 16
 PVD_MAX_FFT_LEN = 2048
+PVD_NOALLOC_MAX_TAPS = 4096
 
 PVD_DPK_BETA = 0
 PVD_DPK_PHOTON = 1
@@ -83,6 +84,7 @@ SIGNATURES = {
     "pvd_batch_run_graph": (_I, [_P, _P]),
     "pvd_batch_destroy": (_I, [_P]),
     "pvd_plan_timings": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
+    "pvd_conv_dose_direct_noalloc": (_I, [ctypes.POINTER(_I), ctypes.POINTER(_I), _I, _P, _P, _P, _P, _D, _P]),
     "pvd_pass_name": (ctypes.c_char_p, [_P, _I]),
     # include/pvd_learned.h
     "pvd_mc_dose": (_I, [ctypes.c_void_p, _P, _P, _P, _P, _P]),
