@@ -2,18 +2,20 @@
 
 Test infrastructure only (see ``oracle/__init__.py``).
 
-Follows ``pyvoxeldosimetry/data/dose_kernels/y90_kernel.py:20-162`` and
-``lu177_kernel.py:52-184`` with the constants of ``Y90/Y90.json:13-25`` and
-``Lu177/Lu177.json:13-27`` (the latter has ``//`` comments, so the reference's
-``json.load`` at ``base_kernel.py:34-36`` cannot read it; the values are the
-ones the file states).
+Follows ``pyvoxeldosimetry/data/dose_kernels/y90_kernel.py:20-162``,
+``lu177_kernel.py:52-184`` and ``ga68_kernel.py:20-106`` with the constants of
+``Y90/Y90.json:13-25``, ``Lu177/Lu177.json:13-27`` and ``Ga68/Ga68.json:13-25``
+(the latter two have ``//`` comments, so the reference's ``json.load`` at
+``base_kernel.py:34-36`` cannot read them; the values are the ones the files state).
 
 Pinned by (tests/test_oracle.py), independently of these formulas' text:
   * the centre voxel: Y-90 raw is NaN (``inf * 0`` at ``y90_kernel.py:134-138``),
     Y-90 fixed centre == rho*spr*f_tissue (1.0 in water), Lu-177 centre ==
-    sum of beta abundances (1.0);
+    sum of beta abundances (1.0); Ga-68 raw NaN, fixed centre == f_tissue;
   * far field beyond every beta range equals the published photon point-source
-    term ``I exp(-mu r/10)/(4 pi r^2)`` at radii computed from integer offsets;
+    term ``I exp(-mu r/10)/(4 pi r^2)`` at radii computed from integer offsets
+    (Ga-68: the annihilation line plus the 1.077 MeV line, times f_tissue); the
+    Ga-68 positron term is zero from 9 E^1.5 f mm on;
   * the grid-centre convention ``g//2`` on even grids (64^3 grid: the NaN sits
     at index 32, and the shipped ``cached_kernels/Y90_water_xy.png`` renders a
     64x64 slice with colour-bar max ~0.90 and a blank (NaN) centre);
@@ -107,6 +109,38 @@ def lu177_kernel(voxel_size: float, grid, tissue: str = "water"):
         g[m] = inten * np.exp(-mu * r[m] / 10) / (4 * np.pi * (r[m] ** 2))
         k += g
     return k
+
+
+# Ga68.json:16-25 and ga68_kernel.py:84-106 (tissue scaling factor; attenuation at 511 keV, 1/cm)
+GA68_BETA_MAX = 1.899
+GA68_ANNIHILATION = 0.511
+GA68_GAMMAS = ((1.077, 0.03),)
+GA68_HALF_LIFE_H = 1.128
+GA68_TISSUE_FACTOR = {"water": 1.0, "lung": 0.3, "soft_tissue": 1.04, "bone": 1.6, "iodine_contrast": 1.3}
+GA68_MU = {"water": 0.096, "lung": 0.029, "soft_tissue": 0.099, "bone": 0.172, "iodine_contrast": 0.158}
+
+
+def ga68_kernel(voxel_size: float, grid, tissue: str = "water", fix_centre: bool = False):
+    """``Ga68KernelGenerator.generate_kernel`` (``ga68_kernel.py:20-106``): positron
+    beta with range ``9 E^1.5 * f`` mm (``:60-62``, the tissue factor multiplies), the
+    511 keV annihilation term and the gamma lines ``I e^{-mu r/10}/(4 pi r^2) (r>0)``
+    with ``mu = mu_511(tissue) (0.511/E)^3.2``, the whole kernel times ``f``.
+    ``fix_centre=False`` keeps the NaN centre (``inf * 0``); ``True`` gives 0 there."""
+    f = GA68_TISSUE_FACTOR.get(tissue, 1.0)
+    mu511 = GA68_MU.get(tissue, 0.096)
+    r = radius_grid(voxel_size, grid)
+    k = np.zeros(tuple(grid))
+    rng = 9.0 * GA68_BETA_MAX ** 1.5 * f
+    m = r <= rng
+    k[m] += (1 - r[m] / rng) ** 2 * np.exp(-2 * r[m] / rng)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for e, inten in ((GA68_ANNIHILATION, 1.0),) + GA68_GAMMAS:
+            mu = mu511 * (0.511 / e) ** 3.2
+            g = inten * np.exp(-mu * r / 10) / (4 * np.pi * r ** 2) * (r > 0)
+            if fix_centre:
+                g[r == 0] = 0.0
+            k += g
+    return k * f
 
 
 def synthetic_kernel(voxel_size: float, grid, length_mm: float = 2.5, core_mm2: float = 0.25):

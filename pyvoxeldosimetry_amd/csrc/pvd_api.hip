@@ -238,7 +238,7 @@ bool register_len(int a, int P, bool fold, int tile) {
 // length.  x: the padded x-planes are implicit zeros and never stored, so a longer
 // x transform costs compute only; y: the y passes and X read and write Py rows;
 // z: the half spectrum H = Pz/2 + 1 sizes every pass.
-constexpr double kRegSlack[3] = {1.0, 0.3, 0.1};
+constexpr double kRegSlack[3] = {1.0, 0.3, 0.2};
 
 int axis_setup(pvd_plan *p) {
     const pvd_plan_params &q = p->prm;

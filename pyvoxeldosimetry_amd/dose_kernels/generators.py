@@ -1,9 +1,12 @@
 """Analytic dose-point-kernel generators evaluated on the GPU in FP64.
 
-Mirrors ``Y90KernelGenerator`` (``pyvoxeldosimetry/data/dose_kernels/y90_kernel.py:7-162``)
-and ``Lu177KernelGenerator`` (``lu177_kernel.py:38-190``): the same kernel
-``K(r)`` on the same grid (centre ``g//2``), built as a list of terms that
-``pvd_dpk_generate`` sums on the device.
+Mirrors ``Y90KernelGenerator`` (``pyvoxeldosimetry/data/dose_kernels/y90_kernel.py:7-162``),
+``Lu177KernelGenerator`` (``lu177_kernel.py:38-190``) and ``Ga68KernelGenerator``
+(``ga68_kernel.py:7-106``): the same kernel ``K(r)`` on the same grid (centre
+``g//2``), built as a list of terms that ``pvd_dpk_generate`` sums on the device.
+Every generator takes an optional ``config_path``: a nuclide JSON file in the
+reference's layout (``base_kernel.py:22-36``), ``//`` comments allowed, whose
+constants replace this package's own.
 """
 from __future__ import annotations
 
@@ -13,7 +16,8 @@ from typing import List, Tuple
 import numpy as np
 
 from .. import _native as nat
-from .nuclides import NUCLIDES, Y90_TISSUE_FACTOR, tissue
+from .config import load_config
+from .nuclides import GA68_MU_511, GA68_TISSUE_FACTOR, NUCLIDES, Y90_TISSUE_FACTOR, nuclide_from_config, tissue
 
 
 def _beta_term(amp: float, range_water_mm: float, rho: float, spr: float) -> nat.DpkTerm:
@@ -24,11 +28,18 @@ def _beta_term(amp: float, range_water_mm: float, rho: float, spr: float) -> nat
 class _DeviceGenerator:
     symbol = ""
 
-    def __init__(self, tissue_type: str):
+    def __init__(self, tissue_type: str, config_path=None):
         if not isinstance(tissue_type, str):
             raise TypeError("tissue_type must be a string")
         self.tissue_type = tissue_type
-        self.nuclide = NUCLIDES[self.symbol]
+        if config_path is not None:
+            self.config = load_config(config_path)
+            self.nuclide = nuclide_from_config(self.config)
+            if self.nuclide.symbol != self.symbol:
+                raise ValueError(f"{config_path} describes {self.nuclide.symbol}, not {self.symbol}")
+        else:
+            self.config = None
+            self.nuclide = NUCLIDES[self.symbol]
         self.tissue_properties = tissue(tissue_type)
 
     def terms(self) -> Tuple[List[nat.DpkTerm], float]:
@@ -66,8 +77,8 @@ class Y90KernelGenerator(_DeviceGenerator):
     """
     symbol = "Y90"
 
-    def __init__(self, tissue_type: str, fix_centre: bool = False):
-        super().__init__(tissue_type)
+    def __init__(self, tissue_type: str, fix_centre: bool = False, config_path=None):
+        super().__init__(tissue_type, config_path)
         self.fix_centre = bool(fix_centre)
 
     def terms(self):
@@ -99,4 +110,32 @@ class Lu177KernelGenerator(_DeviceGenerator):
         return terms, 1.0
 
 
-GENERATORS = {"Y90": Y90KernelGenerator, "Lu177": Lu177KernelGenerator}
+class Ga68KernelGenerator(_DeviceGenerator):
+    """``ga68_kernel.py:20-106``: positron beta + 511 keV annihilation photons + gamma lines,
+    times the tissue scaling factor.
+
+    The positron range is 9 E^1.5 mm times that factor (``:60-62``: multiplied, as the
+    reference does); the photon terms use the tissue's 511 keV coefficient scaled by
+    (0.511/E)^3.2 (``:96-106``).  ``fix_centre=False`` keeps the reference's NaN centre
+    (``exp(0)/(4 pi 0) * (r > 0)`` is ``inf * 0``); ``True`` gives 0 there.
+    """
+    symbol = "Ga68"
+
+    def __init__(self, tissue_type: str, fix_centre: bool = False, config_path=None):
+        super().__init__(tissue_type, config_path)
+        self.fix_centre = bool(fix_centre)
+
+    def terms(self):
+        n = self.nuclide
+        f = GA68_TISSUE_FACTOR.get(self.tissue_type, 1.0)
+        mu511 = GA68_MU_511.get(self.tissue_type, 0.096)
+        centre = 0 if self.fix_centre else nat.PVD_DPK_NAN_CENTRE
+        terms = [nat.DpkTerm(nat.PVD_DPK_BETA, 0, 1.0, n.beta_range_coeff * n.beta_max_mev[0] ** 1.5 * f)]
+        e_ann = n.annihilation_mev or 0.511
+        terms.append(nat.DpkTerm(nat.PVD_DPK_PHOTON, centre, 1.0, mu511 * (0.511 / e_ann) ** 3.2))
+        for e, inten in n.gammas:
+            terms.append(nat.DpkTerm(nat.PVD_DPK_PHOTON, centre, inten, mu511 * (0.511 / e) ** 3.2))
+        return terms, f
+
+
+GENERATORS = {"Y90": Y90KernelGenerator, "Lu177": Lu177KernelGenerator, "Ga68": Ga68KernelGenerator}

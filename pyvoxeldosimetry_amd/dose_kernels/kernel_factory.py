@@ -43,7 +43,7 @@ class KernelFactory:
         if not force_regenerate and key in self._cache:
             return self._cache[key]
         cls = GENERATORS[nuclide]
-        gen = cls(tissue_type, fix_centre=fix_centre) if nuclide == "Y90" else cls(tissue_type)
+        gen = cls(tissue_type, fix_centre=fix_centre) if nuclide in ("Y90", "Ga68") else cls(tissue_type)
         k = gen.generate_kernel_device(voxel_size, grid, device=device)
         self._cache[key] = k
         return k

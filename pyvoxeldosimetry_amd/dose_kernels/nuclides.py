@@ -1,18 +1,25 @@
 """Nuclide and tissue constants of the analytic dose-point kernels.
 
-Values as stated by the reference's data files (tables of physical constants,
-kept identical so kernels match):
-  * ``pyvoxeldosimetry/data/dose_kernels/Y90/Y90.json:13-25`` (half-life 64.1 h,
-    beta max 2.280 MeV, bremsstrahlung on, scaling_factor 1e-12 at ``:31``);
-  * ``Lu177/Lu177.json:13-33`` (half-life 161.52 h, betas, gammas; the file
-    carries ``//`` comments, so these are the values it states);
-  * tissue tables ``y90_kernel.py:57-91`` and ``lu177_kernel.py:76-117``, tissue
-    factors ``y90_kernel.py:148-162``.
+Nuclide constants come from nuclide JSON files in the reference's layout
+(``pyvoxeldosimetry/data/dose_kernels/<Nuclide>/<Nuclide>.json``, read by
+``base_kernel.py:34-36``); this package ships its own copies of the values in
+``dose_kernels/data`` and ``nuclide_from_config`` builds a ``Nuclide`` from any
+such file (``//`` comments allowed, see ``config.py``):
+  * Y-90 (``Y90/Y90.json:13-25``): half-life 64.1 h, beta max 2.280 MeV,
+    bremsstrahlung on, scaling_factor 1e-12;
+  * Lu-177 (``Lu177/Lu177.json:13-33``): half-life 161.52 h, three betas, two
+    gamma lines;
+  * Ga-68 (``Ga68/Ga68.json:13-30``): half-life 1.128 h, beta+ max 1.899 MeV,
+    511 keV annihilation photons, one 1.077 MeV gamma line.
+The tissue tables are the generators' own (``y90_kernel.py:57-91``,
+``lu177_kernel.py:76-117``, ``ga68_kernel.py:84-106``), kept identical so kernels match.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
 from typing import Dict, List, Tuple
+
+from .config import packaged_config
 
 
 @dataclass(frozen=True)
@@ -35,6 +42,17 @@ TISSUES: Dict[str, Tissue] = {
 Y90_TISSUE_FACTOR: Dict[str, float] = {"water": 1.0, "bone": 1.15, "lung": 1.04,
                                        "soft_tissue": 1.04, "iodine_contrast": 1.12}
 
+# Ga-68 tissue scaling of the positron range and of the whole kernel, and linear
+# attenuation at 511 keV in 1/cm; anything else 1.0 / 0.096 (ga68_kernel.py:84-106)
+GA68_TISSUE_FACTOR: Dict[str, float] = {"water": 1.0, "lung": 0.3, "soft_tissue": 1.04, "bone": 1.6,
+                                        "iodine_contrast": 1.3}
+GA68_MU_511: Dict[str, float] = {"water": 0.096, "lung": 0.029, "soft_tissue": 0.099, "bone": 0.172,
+                                 "iodine_contrast": 0.158}
+
+# CSDA range coefficient of each generator's beta model, range = coeff * E^1.5 mm in
+# water (y90_kernel.py:101, lu177_kernel.py:139, ga68_kernel.py:62): code, not file data
+BETA_RANGE_COEFF: Dict[str, float] = {"Y90": 11.0, "Lu177": 5.0, "Ga68": 9.0}
+
 
 @dataclass(frozen=True)
 class Nuclide:
@@ -48,14 +66,36 @@ class Nuclide:
     bremsstrahlung: bool = False
     scaling_factor: float = 1.0e-12
     default_grid: Tuple[int, int, int] = (101, 101, 101)
+    positron: bool = False
+    annihilation_mev: float = 0.0
     extra: Dict[str, float] = field(default_factory=dict)
 
 
-NUCLIDES: Dict[str, Nuclide] = {
-    "Y90": Nuclide("Yttrium-90", "Y90", 64.1, (2.280,), (1.0,), 11.0, (), True, 1.0e-12, (201, 201, 201)),
-    "Lu177": Nuclide("Lutetium-177", "Lu177", 161.52, (0.498, 0.385, 0.177), (0.795, 0.089, 0.116), 5.0,
-                     ((0.208, 0.111), (0.113, 0.062)), False, 1.0e-12, (81, 81, 81)),
-}
+def _as_tuple(v) -> Tuple[float, ...]:
+    return tuple(float(x) for x in (v if isinstance(v, (list, tuple)) else [v]))
+
+
+def nuclide_from_config(cfg: dict) -> Nuclide:
+    """A ``Nuclide`` from a parsed nuclide JSON of the reference's layout."""
+    nuc = cfg["nuclide"]
+    pe = nuc.get("particle_energies", {})
+    symbol = nuc["symbol"]
+    positron = "beta+_max" in pe
+    beta_max = _as_tuple(pe["beta+_max"] if positron else pe["beta-_max"])
+    abund = _as_tuple(pe.get("beta-_abundance", [1.0] * len(beta_max)))
+    if len(abund) != len(beta_max):
+        raise ValueError(f"{symbol}: {len(beta_max)} beta energies but {len(abund)} abundances")
+    gammas = tuple((float(g["energy"]), float(g["intensity"])) for g in pe.get("gamma_lines", []))
+    kern = cfg.get("kernel", {})
+    return Nuclide(name=nuc["name"], symbol=symbol, half_life_h=float(nuc["half_life"]), beta_max_mev=beta_max,
+                   beta_abundance=abund, beta_range_coeff=BETA_RANGE_COEFF.get(symbol, 0.0), gammas=gammas,
+                   bremsstrahlung=bool(pe.get("bremsstrahlung", False)),
+                   scaling_factor=float(kern.get("scaling_factor", 1.0e-12)),
+                   default_grid=tuple(int(g) for g in kern.get("grid_size", (101, 101, 101))),
+                   positron=positron, annihilation_mev=float(pe.get("annihilation", 0.0)))
+
+
+NUCLIDES: Dict[str, Nuclide] = {s: nuclide_from_config(packaged_config(s)) for s in ("Y90", "Lu177", "Ga68")}
 
 SUPPORTED: List[str] = list(NUCLIDES)
 

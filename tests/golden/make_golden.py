@@ -46,6 +46,9 @@ DPK_CASES = [
     ("dpk_lu177_water", "lu177", "water", 2.0, (21, 21, 21), None),
     ("dpk_lu177_lung", "lu177", "lung", 0.98, (15, 15, 15), None),
     ("dpk_lu177_unknown_tissue", "lu177", "liver", 1.0, (7, 7, 7), None),
+    ("dpk_ga68_water_fix", "ga68", "water", 2.0, (25, 25, 25), True),
+    ("dpk_ga68_lung_raw", "ga68", "lung", 1.0, (15, 15, 15), False),
+    ("dpk_ga68_bone_even", "ga68", "bone", 0.98, (10, 12, 8), True),
 ]
 
 
@@ -85,9 +88,11 @@ def build_cases():
     for name, nuc, tissue, voxel, grid, fix in DPK_CASES:
         if nuc == "y90":
             k = dpk.y90_kernel(voxel, grid, tissue, fix_centre=fix)
+        elif nuc == "ga68":
+            k = dpk.ga68_kernel(voxel, grid, tissue, fix_centre=fix)
         else:
             k = dpk.lu177_kernel(voxel, grid, tissue)
-        out[name] = {"expect": k, "nuclide": np.array("Y90" if nuc == "y90" else "Lu177"),
+        out[name] = {"expect": k, "nuclide": np.array({"y90": "Y90", "ga68": "Ga68", "lu177": "Lu177"}[nuc]),
                      "tissue": np.array(tissue), "voxel": np.array(voxel), "grid": np.array(grid),
                      "fix_centre": np.array(bool(fix))}
     return out

@@ -1,4 +1,6 @@
 """Pins the FP64 oracle (CPU only) -- brute force, closed forms, library routines."""
+import math
+
 import numpy as np
 import pytest
 import scipy.signal
@@ -214,3 +216,26 @@ def test_threaded_ref_conv_equals_the_verbatim_expression():
         k = rng.random(kshape)
         np.testing.assert_allclose(conv.ref_conv_threaded(a, k, workers=2), conv.ref_conv(a, k), rtol=0,
                                    atol=1e-12 * np.abs(conv.ref_conv(a, k)).max())
+
+
+def test_ga68_centre_far_field_and_positron_range():
+    raw = dpk.ga68_kernel(1.0, (9, 9, 9))
+    assert np.isnan(raw[4, 4, 4]) and np.isnan(raw).sum() == 1
+    for tissue in ("water", "lung", "bone", "liver"):
+        f = dpk.GA68_TISSUE_FACTOR.get(tissue, 1.0)
+        mu = dpk.GA68_MU.get(tissue, 0.096)
+        k = dpk.ga68_kernel(2.0, (61, 61, 61), tissue, fix_centre=True)
+        assert k[30, 30, 30] == pytest.approx(f, rel=1e-14)           # beta(0) = 1, photons 0 at r = 0
+        rng = 9.0 * 1.899 ** 1.5 * f                                     # 22.3 mm in water
+        for off in ((30, 0, 0), (21, 21, 0), (17, 17, 17), (12, 0, 0)):
+            r = 2.0 * math.sqrt(sum(o * o for o in off))
+            if r <= rng:
+                continue
+            exp = f * (dpk.photon_point_source(r, 1.0, mu) + dpk.photon_point_source(r, 0.03, mu * (0.511 / 1.077) ** 3.2))
+            assert k[30 + off[0], 30 + off[1], 30 + off[2]] == pytest.approx(exp, rel=1e-13), (tissue, off)
+    # just inside the positron range the beta term is (1 - r/R)^2 e^{-2r/R} > 0
+    k = dpk.ga68_kernel(1.0, (61, 61, 61), "water", fix_centre=True)
+    r = 22.0
+    photons = dpk.photon_point_source(r, 1.0, 0.096) + dpk.photon_point_source(r, 0.03, 0.096 * (0.511 / 1.077) ** 3.2)
+    R = 9.0 * 1.899 ** 1.5
+    assert k[30 + 22, 30, 30] - photons == pytest.approx((1 - r / R) ** 2 * math.exp(-2 * r / R), rel=1e-10)
