@@ -103,6 +103,7 @@ SIGNATURES = {
     "pvd_tissue_map": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
     # include/pvd_resample.h
     "pvd_resample": (_I, [_P, _I, _I, _I, ctypes.POINTER(_D), _P, _I, _I, _I, ctypes.POINTER(_D), _I, _D, _P]),
+    "pvd_bspline_prefilter": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     # include/pvd_tac.h
     "pvd_tac_n_params": (_I, [_I]),
     "pvd_tac_fit": (_I, [_P, ctypes.POINTER(_P), ctypes.c_longlong, _P, _P, _P, _P]),

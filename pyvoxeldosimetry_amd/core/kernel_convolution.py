@@ -18,7 +18,9 @@ Config keys read here (all optional):
   ``resample``         True (default): a voxel size other than kernel_resolution
                        is handled as the reference intends (:63-68): activity (and
                        density) are resampled onto the kernel grid, convolved, and
-                       the dose is resampled back (trilinear, GPU).  False: raise.
+                       the dose is resampled back (GPU).  False: raise.
+  ``resample_interpolation`` 'linear' (default), 'cubic' (B-spline) or 'nearest'
+                       for those resampling steps
 """
 from __future__ import annotations
 
@@ -129,10 +131,11 @@ class KernelConvolutionCalculator(DosimetryCalculator):
         """Activity (and density) onto the kernel grid, convolve, dose back onto the input grid."""
         from ..resample import resample_device
         kres = (float(self.kernel_resolution),) * 3
+        interp = self.config.get("resample_interpolation", "linear")
         ak = self._resample_activity(a, voxel_size, kres)
-        dk = None if d is None else resample_device(d, voxel_size, kres, "linear", output_shape=ak.shape)
+        dk = None if d is None else resample_device(d, voxel_size, kres, interp, output_shape=ak.shape)
         dose_k = self.plan_for(ak.shape).dose(ak, density=dk, scale=scale)
-        return resample_device(dose_k, kres, voxel_size, "linear", output_shape=a.shape)
+        return resample_device(dose_k, kres, voxel_size, interp, output_shape=a.shape)
 
     def calculate_dose_rates(self, activity_maps: List, voxel_size, density_map=None, scales=None):
         """One dose-rate map per activity map (the reference's ``dose_rate_maps``)."""
@@ -199,7 +202,8 @@ class KernelConvolutionCalculator(DosimetryCalculator):
         from ..resample import output_shape_for, resample_device
         a, _ = to_device_f32(activity_map, self.kernel_device.device)
         ratio = float(np.prod(output_voxel_size)) / float(np.prod(input_voxel_size))
-        return resample_device(a, input_voxel_size, output_voxel_size, "linear",
+        interp = self.config.get("resample_interpolation", "linear")
+        return resample_device(a, input_voxel_size, output_voxel_size, interp,
                                output_shape=output_shape_for(a.shape, input_voxel_size, output_voxel_size),
                                scale=ratio)
 

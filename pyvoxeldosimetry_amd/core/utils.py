@@ -5,7 +5,9 @@ Mirrors ``pyvoxeldosimetry/core/utils.py``:
 - ``save_dose_map`` / ``load_dose_map`` (``:53-152``): NIfTI-1 dose maps with the
   metadata as a code-44 JSON extension (``pyvoxeldosimetry_amd.io.nifti``
   replaces nibabel);
-- ``verify_file_integrity`` (``:264-301``) and ``get_file_metadata`` (``:303-330``).
+- ``verify_file_integrity`` (``:264-301``) and ``get_file_metadata`` (``:303-330``);
+- ``resample_dose_map`` (``:193-231``): order-3 spline resampling of a dose map,
+  on the GPU (``pyvoxeldosimetry_amd.resample``).
 
 The ``.dat`` layout is the one the reference's reader reads (little-endian):
 ``int32 dims[3]``, ``float32 voxel_size, total_energy, scaling``, ``int32
@@ -185,3 +187,11 @@ def get_file_metadata(filename: Union[str, Path]) -> Dict[str, Any]:
     if ext == ".dat":
         return load_kernel(filename)[1]
     raise ValueError(f"Unsupported file format: {ext}")
+
+
+def resample_dose_map(dose_map, input_spacing: Tuple[float, float, float],
+                      output_spacing: Tuple[float, float, float]):
+    """``resample_dose_map`` (``core/utils.py:193-231``): (resampled dose, affine) with an
+    order-3 spline on the GPU; see ``pyvoxeldosimetry_amd.resample.resample_dose_map``."""
+    from ..resample import resample_dose_map as _rdm
+    return _rdm(dose_map, input_spacing, output_spacing)
