@@ -21,6 +21,7 @@ extern "C" {
 #endif
 
 /* ---------------------------------------------------------------- Monte Carlo */
+#define PVD_MC_MAX_LINES 4
 typedef struct pvd_mc_params {
     int n[3];                 /* grid (x, y, z), z contiguous                         */
     double voxel_mm;          /* isotropic voxel size                                 */
@@ -37,11 +38,28 @@ typedef struct pvd_mc_params {
     double rho_min;           /* densities below are clamped (g/cc)                   */
     const long long *active;  /* device: flat indices of the emitting voxels, or NULL   */
     long long n_active;       /*   (NULL: every voxel, those with activity 0 emit none) */
+    /* Photons, as kerma: each packet leaves the decay point isotropically along a
+     * straight ray and deposits E (1 - exp(-mu_en rho L)) of its remaining energy E
+     * over each voxel chord L (exact voxel traversal), mu_en the energy-absorption
+     * coefficient of water scaled by the local density rho. */
+    int n_lines;                             /* photon lines per decay (<= PVD_MC_MAX_LINES)     */
+    double line_energy[PVD_MC_MAX_LINES];    /* MeV                                              */
+    double line_yield[PVD_MC_MAX_LINES];     /* photons per decay                                */
+    double brems_z;           /* > 0: a bremsstrahlung packet per electron carrying Y(E0) E0, Y the
+                                 thick-target yield 6e-4 Z E/(1 + 6e-4 Z E) of this Z, its mu_en taken
+                                 at k = E0 (1 - sqrt(u)); the electron then deposits (1 - Y) of its
+                                 losses.  0: none                                                */
+    int photon_every;         /* photon packets for every k-th history of a voxel, weight x k (>= 1) */
+    int electrons;            /* 1: transport the beta electrons; 0: photons only                */
+    const float *mu_en;       /* device: water mu_en/rho (cm^2/g) at n_mu log-spaced energies    */
+    int n_mu;
+    double mu_log_e_lo, mu_log_e_hi;  /* ln of the first and last table energy (MeV)          */
 } pvd_mc_params;
 
 /* edep[v] += energy (MeV x activity weight) deposited in voxel v (FP64: a voxel
  * may receive ~1e7 deposits of ~1e-8 of its total); totals[0..2] += emitted,
- * deposited, escaped (FP64).  density may be NULL (water). */
+ * deposited, escaped (FP64; photon energy leaving the grid or below 1e-6 of its
+ * start counts as escaped).  density may be NULL (water). */
 int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const float *density, double *edep,
                 double *totals, void *stream);
 

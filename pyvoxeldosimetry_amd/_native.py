@@ -34,6 +34,7 @@ PATHS = {PVD_PATH_STOCKHAM: "stockham", PVD_PATH_SQ256: "sq256", PVD_PATH_TWO_ST
 PVD_MAX_TIMEPOINTS = 16
 PVD_MAX_FFT_LEN = 2048
 PVD_NOALLOC_MAX_TAPS = 4096
+PVD_MC_MAX_LINES = 4
 
 PVD_DPK_BETA = 0
 PVD_DPK_PHOTON = 1
@@ -140,7 +141,10 @@ class McParams(ctypes.Structure):
                 ("e_of_r", ctypes.c_void_p), ("n_r", ctypes.c_int), ("r_max_mm", ctypes.c_double),
                 ("katz_a", ctypes.c_double), ("katz_b", ctypes.c_double), ("katz_c", ctypes.c_double),
                 ("step_mm", ctypes.c_double), ("scatter", ctypes.c_double), ("rho_min", ctypes.c_double),
-                ("active", ctypes.c_void_p), ("n_active", ctypes.c_longlong)]
+                ("active", ctypes.c_void_p), ("n_active", ctypes.c_longlong), ("n_lines", ctypes.c_int),
+                ("line_energy", ctypes.c_double * 4), ("line_yield", ctypes.c_double * 4), ("brems_z", ctypes.c_double),
+                ("photon_every", ctypes.c_int), ("electrons", ctypes.c_int), ("mu_en", ctypes.c_void_p),
+                ("n_mu", ctypes.c_int), ("mu_log_e_lo", ctypes.c_double), ("mu_log_e_hi", ctypes.c_double)]
 
 
 PVD_CNN_EPI_RELU = 0

@@ -15,10 +15,16 @@
 //    the step starts; steps are at most half a voxel long;
 //  * multiple scattering: after each step the direction turns by a Gaussian polar
 //    angle of width scatter * sqrt(s / R(E0)) about the old direction;
-//  * electrons that leave the grid carry their residual energy out ("escaped").
+//  * electrons that leave the grid carry their residual energy out ("escaped");
+//  * photons (gamma lines per decay, bremsstrahlung packets per electron) as kerma:
+//    isotropic straight rays from the decay point, traversed voxel by voxel
+//    (Amanatides & Woo, Eurographics 1987); over a chord of L mm in a voxel of
+//    density rho the packet deposits E (1 - exp(-mu_en rho L)) of its energy E,
+//    mu_en the water energy-absorption coefficient at its energy.
 // Energy bookkeeping (emitted, deposited, escaped, MeV x weight) is reduced per
-// block in FP64 and added with FP64 atomics; the deposit map uses FP32 atomics.
-// Random numbers: Philox4x32-10 (Salmon et al., SC'11), counter = (history, draw).
+// block in FP64 and added with FP64 atomics; the deposit map uses FP64 atomics.
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11), counter = (history, draw,
+// stream) with separate streams for the electron and the photons.
 #include "pvd_internal.h"
 
 #include "../../include/pvd_learned.h"
@@ -75,7 +81,69 @@ struct MCArgs {
     float rho_min;
     const long long *active;
     long long n_src;          // number of source voxels (active list length or all voxels)
+    int n_lines;
+    float line_e[PVD_MC_MAX_LINES], line_y[PVD_MC_MAX_LINES];
+    float brems_z;
+    int photon_every;
+    int electrons;
+    const float *mu_en;       // water mu_en/rho (cm^2/g) at n_mu log-spaced energies
+    int n_mu;
+    float mu_lo, mu_inv_step; // ln E of entry 0, entries per unit ln E
+    float rho_min_ph;
 };
+
+// mu_en/rho of water at energy e (cm^2/g), log-spaced table, clamped at its ends
+__device__ __forceinline__ float mu_en_of(const MCArgs &a, float e) {
+    return table_lerp(a.mu_en, a.n_mu, (__logf(fmaxf(e, 1e-6f)) - a.mu_lo) * a.mu_inv_step);
+}
+
+// One photon packet carrying energy e (absorbed with mu_en at energy e_mu) and weight w
+// from position p (voxel units) along the unit direction d: exact voxel traversal,
+// kerma deposits; adds to the deposited and escaped totals.
+__device__ void photon_packet(const MCArgs &a, const float *__restrict__ dens, double *__restrict__ edep, float px,
+                              float py, float pz, float dx, float dy, float dz, float e, float e_mu, float w,
+                              double &dep_tot, double &esc_tot) {
+    const float mu_mm = mu_en_of(a, e_mu) * 0.1f;       // 1/mm per g/cc
+    const float voxel_mm = 2.0f * a.half_voxel_mm;
+    int ix = (int)floorf(px), iy = (int)floorf(py), iz = (int)floorf(pz);
+    const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
+    const float big = 3.0e38f;
+    const float tdx = dx != 0.0f ? fabsf(1.0f / dx) : big;
+    const float tdy = dy != 0.0f ? fabsf(1.0f / dy) : big;
+    const float tdz = dz != 0.0f ? fabsf(1.0f / dz) : big;
+    float tmx = dx != 0.0f ? ((dx > 0.0f ? (float)(ix + 1) - px : px - (float)ix) * tdx) : big;
+    float tmy = dy != 0.0f ? ((dy > 0.0f ? (float)(iy + 1) - py : py - (float)iy) * tdy) : big;
+    float tmz = dz != 0.0f ? ((dz > 0.0f ? (float)(iz + 1) - pz : pz - (float)iz) * tdz) : big;
+    float t = 0.0f, erem = e;
+    const float e_stop = 1e-6f * e;
+    const int max_steps = a.n[0] + a.n[1] + a.n[2] + 3;     // a straight ray crosses each plane once
+    for (int step = 0; step < max_steps; ++step) {
+        if (ix < 0 || iy < 0 || iz < 0 || ix >= a.n[0] || iy >= a.n[1] || iz >= a.n[2] || erem <= e_stop) break;
+        const float tn = fminf(tmx, fminf(tmy, tmz));
+        const long long vox = ((long long)ix * a.n[1] + iy) * a.n[2] + iz;
+        const float rho = fmaxf(dens ? dens[vox] : 1.0f, a.rho_min_ph);
+        const float dep = erem * (1.0f - __expf(-mu_mm * rho * (tn - t) * voxel_mm));
+        if (dep > 0.0f) {
+            atomicAdd(&edep[vox], (double)(w * dep));
+            dep_tot += (double)w * dep;
+            erem -= dep;
+        }
+        t = tn;
+        if (tn == tmx) { ix += sx; tmx += tdx; }
+        else if (tn == tmy) { iy += sy; tmy += tdy; }
+        else { iz += sz; tmz += tdz; }
+    }
+    esc_tot += (double)w * erem;
+}
+
+// isotropic unit direction from two uniform draws
+__device__ __forceinline__ void iso_dir(uint32_t u1, uint32_t u2, float &dx, float &dy, float &dz) {
+    const float cth = 2.0f * u01(u1) - 1.0f;
+    const float sth = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
+    float sph, cph;
+    __sincosf(6.28318530718f * u01(u2), &sph, &cph);
+    dx = sth * cph; dy = sth * sph; dz = cth;
+}
 
 __device__ __forceinline__ float range_of(const MCArgs &a, float e) {
     e = fmaxf(e, 1e-4f);
@@ -103,24 +171,55 @@ __global__ __launch_bounds__(256) void pvd_mc_kernel(MCArgs a, const float *__re
         const int vx = (int)(v / ((long long)a.n[2] * a.n[1]));
         // position in voxel units, uniform inside the emitting voxel
         float px = (float)vx + u01(r4.x), py = (float)vy + u01(r4.y), pz = (float)vz + u01(r4.z);
+        const float px0 = px, py0 = py, pz0 = pz;
         float e0 = table_lerp(a.e_of_u, a.n_u, u01(r4.w) * (float)(a.n_u - 1));
         r4 = rng(hid_lo, hid_hi, draw++, 0x5eedu);
-        const float cth = 2.0f * u01(r4.x) - 1.0f;
-        const float sth = sqrtf(fmaxf(0.0f, 1.0f - cth * cth));
-        float sph, cph;
-        __sincosf(6.28318530718f * u01(r4.y), &sph, &cph);
-        float dx = sth * cph, dy = sth * sph, dz = cth;
+        float dx, dy, dz;
+        iso_dir(r4.x, r4.y, dx, dy, dz);
         float r = fminf(range_of(a, e0), a.r_max);
         const float r0 = fmaxf(r, 1e-6f);
         const float dr = a.r_max / (float)(a.n_r - 1);
         const float inv_dr = 1.0f / dr;
         float e_cur = table_lerp(a.e_of_r, a.n_r, r * inv_dr);
-        emitted = (double)w * e_cur;
+        const float e_start = e_cur;
+        // thick-target bremsstrahlung: the electron keeps (1 - Y) of its losses
+        float yb = 0.0f;
+        if (a.brems_z > 0.0f) {
+            const float x = 6.0e-4f * a.brems_z * e_start;
+            yb = x / (1.0f + x);
+        }
+        const float keep = 1.0f - yb;
+        if (a.electrons) emitted = (double)w * keep * e_cur;
+        // photon packets of every photon_every-th history of this voxel, weight x photon_every
+        const int hist = (int)(tid - src * a.histories);
+        if (hist % a.photon_every == 0 && (a.n_lines > 0 || yb > 0.0f)) {
+            const float wp = w * (float)a.photon_every;
+            uint32_t pd = 0;
+            for (int l = 0; l < a.n_lines; ++l) {
+                const uint4 q = rng(hid_lo, hid_hi, pd++, 0xf07u);
+                float qx, qy, qz;
+                iso_dir(q.x, q.y, qx, qy, qz);
+                const float wl = wp * a.line_y[l];
+                emitted += (double)wl * a.line_e[l];
+                photon_packet(a, dens, edep, px0, py0, pz0, qx, qy, qz, a.line_e[l], a.line_e[l], wl, deposited,
+                              escaped);
+            }
+            if (yb > 0.0f) {
+                const uint4 q = rng(hid_lo, hid_hi, pd++, 0xf07u);
+                float qx, qy, qz;
+                iso_dir(q.x, q.y, qx, qy, qz);
+                const float k = fmaxf(e_start * (1.0f - sqrtf(u01(q.z))), 1e-3f);    // energy spectrum ~ (E0 - k)
+                const float eb = yb * e_start;
+                emitted += (double)wp * eb;
+                // the packet carries Y E0 and is absorbed with mu_en at its sampled energy k
+                photon_packet(a, dens, edep, px0, py0, pz0, qx, qy, qz, eb, k, wp, deposited, escaped);
+            }
+        }
         int guard = 0;
-        while (r > 0.0f && guard++ < 100000) {
+        while (a.electrons && r > 0.0f && guard++ < 100000) {
             const int ix = (int)floorf(px), iy = (int)floorf(py), iz = (int)floorf(pz);
             if (ix < 0 || iy < 0 || iz < 0 || ix >= a.n[0] || iy >= a.n[1] || iz >= a.n[2]) {
-                escaped += (double)w * e_cur;
+                escaped += (double)w * keep * e_cur;
                 break;
             }
             const long long vox = ((long long)ix * a.n[1] + iy) * a.n[2] + iz;
@@ -128,7 +227,7 @@ __global__ __launch_bounds__(256) void pvd_mc_kernel(MCArgs a, const float *__re
             const float s = fminf(fminf(a.step_mm, r), a.half_voxel_mm * rho);   // water-equivalent mm
             r = fmaxf(r - s, 0.0f);
             const float e_next = r > 0.0f ? table_lerp(a.e_of_r, a.n_r, r * inv_dr) : 0.0f;
-            const float de = e_cur - e_next;
+            const float de = keep * (e_cur - e_next);
             atomicAdd(&edep[vox], (double)(w * de));
             deposited += (double)w * de;
             e_cur = e_next;
@@ -200,6 +299,19 @@ hipError_t pvd_launch_mc(const pvd_mc_params *p, const float *act, const float *
     a.rho_min = (float)p->rho_min;
     a.active = p->active;
     a.n_src = p->active ? p->n_active : (long long)p->n[0] * p->n[1] * p->n[2];
+    a.n_lines = p->n_lines;
+    for (int l = 0; l < PVD_MC_MAX_LINES; ++l) {
+        a.line_e[l] = l < p->n_lines ? (float)p->line_energy[l] : 0.0f;
+        a.line_y[l] = l < p->n_lines ? (float)p->line_yield[l] : 0.0f;
+    }
+    a.brems_z = (float)p->brems_z;
+    a.photon_every = p->photon_every;
+    a.electrons = p->electrons;
+    a.mu_en = p->mu_en;
+    a.n_mu = p->n_mu;
+    a.mu_lo = (float)p->mu_log_e_lo;
+    a.mu_inv_step = p->n_mu > 1 ? (float)((p->n_mu - 1) / (p->mu_log_e_hi - p->mu_log_e_lo)) : 0.0f;
+    a.rho_min_ph = (float)p->rho_min;
     const long long threads = a.n_src * p->histories;
     if (threads == 0) return hipSuccess;
     const long long blocks = (threads + 255) / 256;
@@ -219,6 +331,16 @@ extern "C" int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const 
     if (!(p->voxel_mm > 0.0) || !(p->step_mm > 0.0) || !(p->rho_min > 0.0))
         return pvd_set_error(PVD_EINVAL, "voxel_mm, step_mm and rho_min must be positive");
     if (p->active && p->n_active < 0) return pvd_set_error(PVD_EINVAL, "negative n_active");
+    if (p->n_lines < 0 || p->n_lines > PVD_MC_MAX_LINES)
+        return pvd_set_error(PVD_EINVAL, "n_lines must be in [0, %d]", PVD_MC_MAX_LINES);
+    for (int l = 0; l < p->n_lines; ++l)
+        if (!(p->line_energy[l] > 0.0) || !(p->line_yield[l] >= 0.0))
+            return pvd_set_error(PVD_EINVAL, "photon line %d needs energy > 0 and yield >= 0", l);
+    if (p->photon_every < 1) return pvd_set_error(PVD_EINVAL, "photon_every must be >= 1");
+    if (!(p->brems_z >= 0.0)) return pvd_set_error(PVD_EINVAL, "brems_z must be >= 0");
+    if ((p->n_lines > 0 || p->brems_z > 0.0) &&
+        (!p->mu_en || p->n_mu < 2 || !(p->mu_log_e_hi > p->mu_log_e_lo)))
+        return pvd_set_error(PVD_EINVAL, "photons need the mu_en table");
     const hipError_t e = pvd_launch_mc(p, activity, density, edep, totals, (hipStream_t)stream);
     if (e != hipSuccess) return pvd_hip_error(e, "launch Monte Carlo");
     return PVD_OK;

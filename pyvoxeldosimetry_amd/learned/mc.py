@@ -3,6 +3,8 @@
 Produces the heterogeneous-medium dose targets of the learned correction and
 the Monte-Carlo-derived point kernel that the plain convolution uses in the
 learned pipeline, so that homogeneous water is consistent between the two.
+Beta electrons (condensed history) and, by default, the nuclide's photons
+(Lu-177 gamma lines, Y-90 bremsstrahlung) as kerma packets.
 """
 from __future__ import annotations
 
@@ -16,7 +18,12 @@ from . import spectra
 
 class MonteCarlo:
     def __init__(self, nuclide: str = "Y90", step_mm: float = 0.5, scatter: float = 0.6, rho_min: float = 1e-3,
-                 device=None):
+                 device=None, photons: bool = True, photon_every: int = 16, photon_lines=None,
+                 brems_z=None, electrons: bool = True):
+        """``photons``: the nuclide's gamma lines and bremsstrahlung (``spectra.PHOTON_LINES``,
+        ``spectra.BREMS_Z``) as kerma packets, for every ``photon_every``-th history of a
+        voxel (weight x photon_every); ``photon_lines`` / ``brems_z`` override them;
+        ``electrons=False`` transports photons only."""
         import torch
         nat.require_device()
         if nuclide not in spectra.BETA_SPECTRA:
@@ -29,6 +36,15 @@ class MonteCarlo:
         self.e_of_u = torch.tensor(e_u, dtype=torch.float32, device=self.device)
         self.e_of_r = torch.tensor(e_r, dtype=torch.float32, device=self.device)
         self.step_mm, self.scatter, self.rho_min = float(step_mm), float(scatter), float(rho_min)
+        lines = spectra.PHOTON_LINES[nuclide] if photon_lines is None else tuple(photon_lines)
+        self.photon_lines = tuple(lines) if photons else ()
+        if len(self.photon_lines) > nat.PVD_MC_MAX_LINES:
+            raise ValueError(f"at most {nat.PVD_MC_MAX_LINES} photon lines")
+        self.brems_z = (spectra.BREMS_Z[nuclide] if brems_z is None else float(brems_z)) if photons else 0.0
+        self.photon_every = int(photon_every)
+        self.electrons = bool(electrons)
+        mu, self.mu_lo, self.mu_hi = spectra.mu_en_table()
+        self.mu_en = torch.tensor(mu, dtype=torch.float32, device=self.device)
         self.last_totals = None
 
     def edep(self, activity, density, voxel_mm: float, histories: int = 64, seed: int = 1, out=None):
@@ -53,6 +69,14 @@ class MonteCarlo:
         p.r_max_mm = self.r_max
         p.katz_a, p.katz_b, p.katz_c = spectra.KATZ_A, spectra.KATZ_B, spectra.KATZ_C
         p.step_mm, p.scatter, p.rho_min = self.step_mm, self.scatter, self.rho_min
+        p.n_lines = len(self.photon_lines)
+        for i, (e, y) in enumerate(self.photon_lines):
+            p.line_energy[i], p.line_yield[i] = float(e), float(y)
+        p.brems_z = self.brems_z
+        p.photon_every = self.photon_every
+        p.electrons = int(self.electrons)
+        p.mu_en, p.n_mu = self.mu_en.data_ptr(), self.mu_en.numel()
+        p.mu_log_e_lo, p.mu_log_e_hi = self.mu_lo, self.mu_hi
         # emitting voxels only: one thread per (emitting voxel, history)
         active = torch.nonzero(a.reshape(-1) > 0).reshape(-1).contiguous()
         p.active, p.n_active = active.data_ptr(), active.numel()

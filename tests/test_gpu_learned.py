@@ -263,6 +263,110 @@ def test_mc_point_kernel_is_isotropic_and_consistent():
     assert np.all(np.diff(prof[:6]) < 0)
 
 
+def test_mc_electrons_deposit_nothing_beyond_the_csda_range_of_emax():
+    """Electron path lengths are the Katz-Penfold CSDA range R(E0) <= R(E_max) and a
+    deposit goes to the voxel where its step starts, so no voxel farther than
+    R(E_max) from the emitting voxel receives energy (exact, no statistics)."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import BETA_SPECTRA, katz_range_mm
+    _say("mc csda range")
+    for nuc, v in (("Y90", 1.0), ("Lu177", 0.25)):
+        mc = MonteCarlo(nuc, photons=False)
+        n = 31
+        a = torch.zeros((n, n, n), dtype=torch.float32, device="cuda")
+        c = n // 2
+        a[c, c, c] = 1.0
+        e = mc.edep(a, None, v, histories=1 << 18, seed=5).cpu().numpy()
+        rmax = float(katz_range_mm(BETA_SPECTRA[nuc][0][-1]))
+        g = np.abs(np.arange(n) - c)
+        gap = np.maximum(g - 1, 0) * v                      # nearest distance between voxel boxes, per axis
+        dmin = np.sqrt(gap[:, None, None] ** 2 + gap[None, :, None] ** 2 + gap[None, None, :] ** 2)
+        assert e[dmin > rmax].max() == 0.0, nuc
+        assert e[dmin > 0.5 * rmax].max() > 0.0, nuc             # the range is actually approached
+
+
+def test_mc_mean_deposit_per_decay_is_the_spectrum_mean_in_an_infinite_medium():
+    """Electrons only, a water cube wider than twice the largest range: nothing escapes
+    and the deposited energy per decay is the beta spectrum's mean energy."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import mean_energy
+    _say("mc mean deposit")
+    for nuc in ("Y90", "Lu177"):
+        mc = MonteCarlo(nuc, photons=False)
+        a = torch.zeros((15, 15, 15), dtype=torch.float32, device="cuda")
+        a[7, 7, 7] = 1.0
+        mc.edep(a, None, 2.0, histories=1 << 20, seed=11)
+        emitted, deposited, escaped = mc.last_totals.cpu().numpy()
+        assert escaped == 0.0, nuc
+        assert deposited == pytest.approx(mean_energy(nuc), rel=3e-3), nuc
+
+
+def _escape_expectation(mu_per_voxel, n, samples=1 << 20, seed=0):
+    """E[exp(-mu s)] over isotropic directions from points uniform in the centre voxel
+    of an n^3 cube, s the distance to the cube's surface (independent numpy estimate)."""
+    rng = np.random.default_rng(seed)
+    p = n // 2 + rng.random((samples, 3))
+    d = rng.normal(size=(samples, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    with np.errstate(divide="ignore"):
+        t = np.where(d > 0, (n - p) / d, np.where(d < 0, -p / d, np.inf))
+    return float(np.exp(-mu_per_voxel * t.min(axis=1)).mean())
+
+
+def test_mc_photon_kerma_escape_and_radial_profile_match_closed_forms():
+    """Photons only (one 30 keV line, mu_en = 0.1557 cm^2/g): the escaped fraction equals
+    E[exp(-mu_en s_exit)] and the deposit density at radius r follows
+    mu_en exp(-mu_en r) / (4 pi r^2) (kerma of a point source)."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import water_mu_en
+    _say("mc photon closed forms")
+    mc = MonteCarlo("Y90", electrons=False, photon_lines=[(0.03, 1.0)], brems_z=0.0, photon_every=1)
+    n, v = 61, 1.0
+    a = torch.zeros((n, n, n), dtype=torch.float32, device="cuda")
+    c = n // 2
+    a[c, c, c] = 1.0
+    e = mc.edep(a, None, v, histories=1 << 20, seed=3).cpu().numpy()
+    emitted, deposited, escaped = mc.last_totals.cpu().numpy()
+    assert emitted == pytest.approx(0.03, rel=1e-6)
+    assert deposited + escaped == pytest.approx(emitted, rel=1e-5)
+    mu = float(water_mu_en(0.03)) * 0.1 * v                 # per voxel
+    assert escaped / emitted == pytest.approx(_escape_expectation(mu, n), abs=2e-3)
+    g = np.arange(n) - c
+    r = np.sqrt(g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2) * v
+    for r0 in (8.0, 14.0, 20.0):
+        shell = (r > r0 - 1.0) & (r <= r0 + 1.0)
+        got = e[shell].sum() / (shell.sum() * v ** 3)
+        exp = 0.03 * (mu / v) * (np.exp(-(mu / v) * r[shell]) / (4 * np.pi * r[shell] ** 2)).mean()
+        assert got == pytest.approx(exp, rel=0.05), r0
+
+
+def test_mc_lu177_gamma_lines_and_y90_bremsstrahlung_energy_budget():
+    """With photons, emitted energy per decay = beta mean + sum of line energy x yield
+    (Lu-177), or stays the beta mean with Y(E0) of it carried by bremsstrahlung (Y-90);
+    deposited + escaped = emitted for every configuration."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.mc import MonteCarlo
+    from pyvoxeldosimetry_amd.learned.spectra import PHOTON_LINES, mean_energy
+    _say("mc photon budget")
+    rng = np.random.default_rng(2)
+    a = torch.from_numpy(rng.uniform(0, 1, size=(24, 20, 18)).astype(np.float32)).cuda()
+    rho = torch.from_numpy(rng.choice([0.3, 1.0, 1.85], size=(24, 20, 18)).astype(np.float32)).cuda()
+    decays = a.double().sum().item()
+    for nuc in ("Lu177", "Y90"):
+        mc = MonteCarlo(nuc)
+        mc.edep(a, rho, 2.0, histories=256, seed=4)
+        emitted, deposited, escaped = mc.last_totals.cpu().numpy()
+        assert abs(deposited + escaped - emitted) / emitted < 1e-5, nuc
+        want = mean_energy(nuc) + sum(e * y for e, y in PHOTON_LINES[nuc])
+        assert emitted / decays == pytest.approx(want, rel=1.5e-2), nuc
+        nophot = MonteCarlo(nuc, photons=False)
+        e_np = nophot.edep(a, rho, 2.0, histories=256, seed=4)
+        assert e_np.sum().item() < deposited                  # photons add deposits
+
+
 # ----------------------------------------------------------------- learned pipeline
 def test_correction_fallback_rule():
     """Out-of-bound corrections injected from the host fall back to the plain dose."""

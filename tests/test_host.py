@@ -186,7 +186,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         pytest.skip("gcc not available")
     fields = {"pvd_plan_params": (nat.PlanParams, ["flags", "tile", "max_radix"]),
               "pvd_plan_info": (nat.PlanInfo, ["workspace_bytes", "tile", "max_radix", "fft_path", "block", "grid"]),
-              "pvd_dpk_term": (nat.DpkTerm, ["param"]), "pvd_mc_params": (nat.McParams, ["active", "n_active"]),
+              "pvd_dpk_term": (nat.DpkTerm, ["param"]), "pvd_mc_params": (nat.McParams, ["active", "n_active", "line_energy", "brems_z", "mu_en", "mu_log_e_hi"]),
               "pvd_tissue_params": (nat.TissueParams, ["sigma", "truncate"]),
               "pvd_tac_params": (nat.TacParams, ["tol", "fail_params", "t_int"])}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pvd.h"', '#include "pvd_learned.h"',
