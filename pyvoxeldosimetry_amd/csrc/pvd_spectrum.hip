@@ -93,7 +93,14 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// T1[a][b][kz], a < ux, b < uy, kz < H
+// (q + d) mod P for q, d in [0, P)
+__device__ __forceinline__ int add_mod(int q, int d, int P) {
+    q += d;
+    return q >= P ? q - P : q;
+}
+
+// T1[a][b][kz], a < ux, b < uy, kz < H: the twiddle index kz * pos(m) mod P advances
+// by kz (mod P) from one tap to the next (pos(m + 1) = pos(m) + 1 mod P)
 __global__ __launch_bounds__(256) void pvd_spec_z(pvd_spec_args p, const double *__restrict__ K,
                                                   double2 *__restrict__ T1) {
     extern __shared__ double2 tw64[];
@@ -107,12 +114,14 @@ __global__ __launch_bounds__(256) void pvd_spec_z(pvd_spec_args p, const double 
         const int a = (int)(idx / ((long long)p.H * p.a[1].u));
         const double *row = K + ((long long)(a + p.a[0].lo) * p.K[1] + (b + p.a[1].lo)) * p.K[2];
         double2 acc = make_double2(0.0, 0.0);
+        const int dq = kz % az.P;
+        int q = (int)(((long long)kz * spec_pos(az, az.lo)) % az.P);
         for (int m = 0; m < az.u; ++m) {
-            const int mz = m + az.lo;
-            const double kv = row[mz];
-            const double2 w = tw64[(int)(((long long)kz * spec_pos(az, mz)) % az.P)];
+            const double kv = row[m + az.lo];
+            const double2 w = tw64[q];
             acc.x += kv * w.x;
             acc.y += kv * w.y;
+            q = add_mod(q, dq, az.P);
         }
         T1[idx] = acc;
     }
@@ -131,12 +140,13 @@ __global__ __launch_bounds__(256) void pvd_spec_y(pvd_spec_args p, const double2
         const int ky = (int)((idx / p.H) % ay.P);
         const int a = (int)(idx / ((long long)p.H * ay.P));
         double2 acc = make_double2(0.0, 0.0);
+        int q = (int)(((long long)ky * spec_pos(ay, ay.lo)) % ay.P);
         for (int b = 0; b < ay.u; ++b) {
             const double2 t = T1[((long long)a * ay.u + b) * p.H + kz];
-            const double2 w = tw64[(int)(((long long)ky * spec_pos(ay, b + ay.lo)) % ay.P)];
-            const double2 r = zmul(t, w);
+            const double2 r = zmul(t, tw64[q]);
             acc.x += r.x;
             acc.y += r.y;
+            q = add_mod(q, ky, ay.P);
         }
         T2[idx] = acc;
     }
@@ -154,22 +164,96 @@ __global__ __launch_bounds__(256) void pvd_spec_x(pvd_spec_args p, const double2
          idx += (long long)gridDim.x * blockDim.x) {
         const long long c = idx % plane;
         const int kx = (int)(idx / plane);
-        const long long ky = c / p.H, kz = c - ky * p.H;
-        if (p.sym && (kx > ax.P / 2 || ky > p.a[1].P / 2)) continue;
         double2 acc = make_double2(0.0, 0.0);
+        int q = (int)(((long long)kx * spec_pos(ax, ax.lo)) % ax.P);
         for (int a = 0; a < ax.u; ++a) {
-            const double2 t = T2[(long long)a * plane + c];
-            const double2 w = tw64[(int)(((long long)kx * spec_pos(ax, a + ax.lo)) % ax.P)];
-            const double2 r = zmul(t, w);
+            const double2 r = zmul(T2[(long long)a * plane + c], tw64[q]);
             acc.x += r.x;
             acc.y += r.y;
+            q = add_mod(q, kx, ax.P);
         }
-        if (p.sym) {
-            float *sr = reinterpret_cast<float *>(S);
-            sr[((long long)kx * (p.a[1].P / 2 + 1) + ky) * p.Hp + kz] = (float)(acc.x * norm);
-        } else {
-            S[((long long)kx * p.a[1].P + ky) * p.Hp + kz] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
+        const long long ky = c / p.H, kz = c - ky * p.H;
+        S[((long long)kx * p.a[1].P + ky) * p.Hp + kz] = make_float2((float)(acc.x * norm), (float)(acc.y * norm));
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Mirror-symmetric kernel anchored at its centre c = K//2 (s = c, lo = 0, u = K, K odd,
+// K[c + j] == K[c - j] checked exactly on the host): every axis transform is the real
+// cosine sum  sum_j w_j v[c + j] cos(2 pi k j / P),  w_0 = 1, w_j = 2 (j = 1..c),
+// over the half taps j = 0..c, and only the octant kx <= Px/2, ky <= Py/2 is needed.
+//   R1[a][b][kz]  a, b in 0..c   (kz < H)
+//   R2[a][ky][kz] ky <= Py/2
+//   S[kx][ky][kz] kx <= Px/2     (FP32 real octant, rows of pitch Hp)
+__device__ __forceinline__ void fill_cos64(double *cs, int P) {
+    for (int q = threadIdx.x; q < P; q += blockDim.x) cs[q] = cospi(2.0 * (double)q / (double)P);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void pvd_spec_z_sym(pvd_spec_args p, const double *__restrict__ K,
+                                                      double *__restrict__ R1) {
+    extern __shared__ double cs64[];
+    const int P = p.a[2].P, cz = p.K[2] / 2, cy = p.K[1] / 2, cx = p.K[0] / 2;
+    fill_cos64(cs64, P);
+    const long long total = (long long)(cx + 1) * (cy + 1) * p.H;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(idx % p.H);
+        const int b = (int)((idx / p.H) % (cy + 1));
+        const int a = (int)(idx / ((long long)p.H * (cy + 1)));
+        const double *row = K + ((long long)(cx + a) * p.K[1] + (cy + b)) * p.K[2] + cz;
+        const int dq = kz % P;
+        double acc = row[0];
+        int q = dq;
+        for (int j = 1; j <= cz; ++j) {
+            acc += 2.0 * row[j] * cs64[q];
+            q = add_mod(q, dq, P);
         }
+        R1[idx] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void pvd_spec_y_sym(pvd_spec_args p, const double *__restrict__ R1,
+                                                      double *__restrict__ R2) {
+    extern __shared__ double cs64[];
+    const int P = p.a[1].P, cy = p.K[1] / 2, cx = p.K[0] / 2, hy = P / 2 + 1;
+    fill_cos64(cs64, P);
+    const long long total = (long long)(cx + 1) * hy * p.H;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(idx % p.H);
+        const int ky = (int)((idx / p.H) % hy);
+        const int a = (int)(idx / ((long long)p.H * hy));
+        const double *col = R1 + (long long)a * (cy + 1) * p.H + kz;
+        double acc = col[0];
+        int q = ky;
+        for (int j = 1; j <= cy; ++j) {
+            acc += 2.0 * col[(long long)j * p.H] * cs64[q];
+            q = add_mod(q, ky, P);
+        }
+        R2[idx] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void pvd_spec_x_sym(pvd_spec_args p, const double *__restrict__ R2, double norm,
+                                                      float *__restrict__ S) {
+    extern __shared__ double cs64[];
+    const int P = p.a[0].P, cx = p.K[0] / 2, hx = P / 2 + 1, hy = p.a[1].P / 2 + 1;
+    fill_cos64(cs64, P);
+    const long long plane = (long long)hy * p.H;
+    const long long total = (long long)hx * plane;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long c = idx % plane;
+        const int kx = (int)(idx / plane);
+        double acc = R2[c];
+        int q = kx;
+        for (int j = 1; j <= cx; ++j) {
+            acc += 2.0 * R2[(long long)j * plane + c] * cs64[q];
+            q = add_mod(q, kx, P);
+        }
+        const long long ky = c / p.H, kz = c - ky * p.H;
+        S[((long long)kx * hy + ky) * p.Hp + kz] = (float)(acc * norm);
     }
 }
 
@@ -179,8 +263,10 @@ static int grid_for(long long total, int block) {
     return (int)(g > 0 ? g : 1);
 }
 
-// T1/T2 are caller-provided FP64 scratch of ux*uy*H and ux*Py*H double2.  With
-// sym, S receives the real octant [Px/2+1][Py/2+1][Hp] as FP32 scalars.
+// Scratch (FP64, caller-provided): general kernels T1 = ux*uy*H and T2 = ux*Py*H
+// double2; symmetric kernels (sym, centre-anchored) R1 = (cx+1)(cy+1)H and
+// R2 = (cx+1)(Py/2+1)H doubles, in the same two buffers.  With sym, S receives the
+// real octant [Px/2+1][Py/2+1][Hp] as FP32 scalars.
 hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], const int s[3], const int K[3],
                                int Hp, int sym, const double *kernel, double2 *T1, double2 *T2, float2 *S,
                                hipStream_t st) {
@@ -193,15 +279,28 @@ hipError_t pvd_launch_spectrum(const int P[3], const int lo[3], const int u[3], 
     p.Hp = Hp;
     p.sym = sym;
     const int block = 256;
+    const double norm = 1.0 / ((double)P[0] * (double)P[1] * (double)P[2]);
+    hipError_t e;
+    if (sym) {
+        const long long cx = K[0] / 2, cy = K[1] / 2;
+        double *R1 = reinterpret_cast<double *>(T1), *R2 = reinterpret_cast<double *>(T2);
+        hipLaunchKernelGGL(pvd_spec_z_sym, dim3(grid_for((cx + 1) * (cy + 1) * p.H, block)), dim3(block),
+                           sizeof(double) * P[2], st, p, kernel, R1);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(pvd_spec_y_sym, dim3(grid_for((cx + 1) * (P[1] / 2 + 1) * p.H, block)), dim3(block),
+                           sizeof(double) * P[1], st, p, (const double *)R1, R2);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(pvd_spec_x_sym, dim3(grid_for((long long)(P[0] / 2 + 1) * (P[1] / 2 + 1) * p.H, block)),
+                           dim3(block), sizeof(double) * P[0], st, p, (const double *)R2, norm,
+                           reinterpret_cast<float *>(S));
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(pvd_spec_z, dim3(grid_for((long long)u[0] * u[1] * p.H, block)), dim3(block),
                        sizeof(double2) * P[2], st, p, kernel, T1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(pvd_spec_y, dim3(grid_for((long long)u[0] * P[1] * p.H, block)), dim3(block),
                        sizeof(double2) * P[1], st, p, T1, T2);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const double norm = 1.0 / ((double)P[0] * (double)P[1] * (double)P[2]);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(pvd_spec_x, dim3(grid_for((long long)P[0] * P[1] * p.H, block)), dim3(block),
                        sizeof(double2) * P[0], st, p, T2, norm, S);
     return hipGetLastError();

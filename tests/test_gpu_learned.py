@@ -363,8 +363,11 @@ def test_mc_lu177_gamma_lines_and_y90_bremsstrahlung_energy_budget():
         want = mean_energy(nuc) + sum(e * y for e, y in PHOTON_LINES[nuc])
         assert emitted / decays == pytest.approx(want, rel=1.5e-2), nuc
         nophot = MonteCarlo(nuc, photons=False)
-        e_np = nophot.edep(a, rho, 2.0, histories=256, seed=4)
-        assert e_np.sum().item() < deposited                  # photons add deposits
+        e_np = nophot.edep(a, rho, 2.0, histories=256, seed=4).sum().item()
+        if nuc == "Lu177":
+            assert e_np < deposited                            # the gamma lines add deposits
+        else:                                                  # bremsstrahlung carries ~0.4 % away
+            assert 0.99 * e_np < deposited < e_np
 
 
 # ----------------------------------------------------------------- learned pipeline
