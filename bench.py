@@ -274,7 +274,11 @@ def main():
             "ratio_corrected_over_plain": round(m["ms_corr"] / m["ms_plain"], 2),
             "cnn_forward_ms": round(m["ms_net"], 4),
             "mfma_frac": round(m["alg_flops"] / (m["ms_net"] * 1e-3) / 2.5e15, 4),
-            "workload": "Y-90 128^3 @2 mm, plain FFT dose + 6-layer 3^3 residual CNN (BF16 MFMA) + bounded correction",
+            "region_mode_ms": round(m["ms_corr_region"], 4),
+            "region_mode_block_fraction": round(m["region_block_frac"], 4),
+            "workload": "Y-90 128^3 @2 mm (synth_phantom seed 123), plain FFT dose + 6-layer 3^3 residual CNN "
+                        "(BF16 MFMA) over the whole volume + bounded correction (region mode: only the blocks "
+                        "within the receptive field of plain dose >= 5 % of max)",
             "error_vs_monte_carlo": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ev.items()},
             "error_sample": "3 held-out phantoms (seeds 20000..20002) at 64^3, Monte Carlo 4096 histories/voxel; "
                             "voxels >= 10 % of max"}
@@ -425,6 +429,10 @@ def cnn_measure(dev, steps: int, warmup: int, eval_seeds=None, eval_shape=(64, 6
     act, rho = synth_phantom(123, shape, dev)
     ms_plain = _timed(lambda: corr.plain(act, rho), steps, warmup)
     ms_corr = _timed(lambda: corr.correct(act, rho), max(1, steps // 5), warmup)
+    ms_corr_region = _timed(lambda: corr.correct(act, rho, region=True), max(1, steps // 5), warmup)
+    dp = corr.plain(act, rho)
+    region_blocks = corr.region_blocks(dp, dp.max().reshape(1), corr.region_frac)[1].item()
+    all_blocks = int(corr.lib.pvd_cnn_region_block_count(*shape))
     x, _ = corr.pack(act, rho, corr.plain(act, rho))
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     reps = 20
@@ -440,7 +448,8 @@ def cnn_measure(dev, steps: int, warmup: int, eval_seeds=None, eval_shape=(64, 6
     executed = 2.0 * 27 * (16 * 32 + 4 * 32 * 32 + 32 * 32) * n    # as issued (16/32-channel MFMA tiles)
     ev = None if eval_seeds is None else evaluate(corr, eval_seeds, eval_shape, histories=histories)
     return {"corr": corr, "ms_plain": ms_plain, "ms_corr": ms_corr, "ms_net": ms_net, "alg_flops": alg,
-            "executed_flops": executed, "eval": ev, "act": act}
+            "executed_flops": executed, "eval": ev, "act": act, "ms_corr_region": ms_corr_region,
+            "region_block_frac": region_blocks / all_blocks}
 
 
 def bench_cnn(args):
@@ -464,7 +473,9 @@ def bench_cnn(args):
            "config": {"workload": "plain FFT dose (linear, 13^3 MC kernel) + 6-layer 3^3 residual CNN "
                                   "(32 ch, BF16 MFMA, FP32 accumulate) + bounded correction, 128^3",
                       "plain_ms": round(ms_plain, 4), "cnn_forward_ms": round(ms_net, 4),
-                      "ratio_corrected_over_plain": round(ms_corr / ms_plain, 2)},
+                      "ratio_corrected_over_plain": round(ms_corr / ms_plain, 2),
+                      "region_mode_ms": round(m["ms_corr_region"], 4),
+                      "region_mode_block_fraction": round(m["region_block_frac"], 4)},
            "roofline": {"bound": "mfma", "achieved": round(alg / (ms_net * 1e-3) / 1e12, 1), "peak": peak,
                         "unit": "TFLOP/s", "frac": round(alg / (ms_net * 1e-3) / 1e12 / peak, 4),
                         "executed_TFLOPs": round(executed / (ms_net * 1e-3) / 1e12, 1),

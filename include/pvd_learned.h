@@ -76,8 +76,21 @@ int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const float *dens
 #define PVD_CNN_EPI_LINEAR_F32  2  /* y32[v] = z[v][0] + b[0]  (FP32, channel 0 only)    */
 #define PVD_CNN_EPI_ACCUM_F32   3  /* y32[v][c] += z[v][c]      (FP32, all 32 channels)   */
 
+/* blocks / nblocks (device, both NULL = every block): compute only the listed blocks of
+ * 2 x 4 x 32 output voxels (block index (bx * nby + by) * nbz + bz with nby = ceil(n1/4),
+ * nbz = n2/32; *nblocks entries), e.g. from pvd_cnn_region_blocks. */
 int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, int epi, const void *resid,
-                     void *y, void *act, float *y32, int n0, int n1, int n2, void *stream);
+                     void *y, void *act, float *y32, int n0, int n1, int n2, const int *blocks, const int *nblocks,
+                     void *stream);
+
+/* The blocks within the network's 6-voxel receptive field of the region dp >= frac *
+ * (*dose_max): block_list receives their indices (in no fixed order) and *nblocks their
+ * number, both on the device (no host synchronisation).  flags: int scratch of
+ * pvd_cnn_region_block_count() entries, as is block_list.  A forward pass over these
+ * blocks gives the network's output exactly on the region (elsewhere it is undefined). */
+size_t pvd_cnn_region_block_count(int n0, int n1, int n2);
+int pvd_cnn_region_blocks(const float *dp, int n0, int n1, int n2, const float *dose_max, float frac, int *flags,
+                          int *block_list, int *nblocks, void *stream);
 
 /* Weights of the transposed convolution used for the input gradient:
  * wt[tap][ci][co] = w[26 - tap][co][ci] (cin = 32). */
@@ -113,11 +126,13 @@ int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *para
 int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, const float *act_max,
                         const float *dose_max, void *x, long long n_voxels, void *stream);
 
-/* Corrected dose with fallback: out = dp*(1+r) where |r| <= bound, else dp.
+/* Corrected dose with fallback: out = dp*(1+r) where dp >= apply_frac * (*dose_max) and
+ * |r| <= bound, else dp (r is read only where applied: 0 = everywhere).
  * counts[0] += voxels with |r| > bound and dp >= region_frac * (*dose_max);
  * counts[1] += voxels with dp >= region_frac * (*dose_max). */
 int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const float *dose_max, float region_frac,
-                             float *out, unsigned long long *counts, long long n_voxels, void *stream);
+                             float apply_frac, float *out, unsigned long long *counts, long long n_voxels,
+                             void *stream);
 
 /* Sum rows of a [rows][cols] FP32 array in fixed order into out[cols] (+= if accumulate). */
 int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream);

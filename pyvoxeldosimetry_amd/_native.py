@@ -89,7 +89,9 @@ SIGNATURES = {
     "pvd_pass_name": (ctypes.c_char_p, [_P, _I]),
     # include/pvd_learned.h
     "pvd_mc_dose": (_I, [ctypes.c_void_p, _P, _P, _P, _P, _P]),
-    "pvd_cnn_conv_fwd": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P]),
+    "pvd_cnn_conv_fwd": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
+    "pvd_cnn_region_block_count": (ctypes.c_size_t, [_I, _I, _I]),
+    "pvd_cnn_region_blocks": (_I, [_P, _I, _I, _I, _P, ctypes.c_float, _P, _P, _P, _P]),
     "pvd_cnn_flip_weights": (_I, [_P, _P, _P]),
     "pvd_cnn_relu_grad": (_I, [_P, _P, _P, _P, ctypes.c_longlong, _P]),
     "pvd_cnn_wgrad_work_floats": (ctypes.c_size_t, [_I, _I, _I]),
@@ -99,7 +101,8 @@ SIGNATURES = {
                           ctypes.c_float, _I, _P]),
     "pvd_reduce_rows": (_I, [_P, _I, _I, _P, _I, _P]),
     "pvd_cnn_pack_inputs": (_I, [_P, _P, _P, _P, _P, _P, ctypes.c_longlong, _P]),
-    "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, _P, ctypes.c_float, _P, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, _P, ctypes.c_float, ctypes.c_float, _P, _P,
+                                      ctypes.c_longlong, _P]),
     # include/pvd_tissue.h
     "pvd_tissue_map": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
     # include/pvd_resample.h

@@ -14,6 +14,8 @@
 //           three times, pre-shifted by dz = -1, 0, +1, so every fragment read is a
 //           16-byte aligned load.  Each wave owns a fixed subset of taps; per-
 //           workgroup partial sums are reduced in a fixed order (deterministic).
+#include <algorithm>
+
 #include "pvd_internal.h"
 
 #include "../../include/pvd_learned.h"
@@ -50,6 +52,8 @@ struct ConvArgs {
     uint16_t *act;
     float *y32;
     int n0, n1, n2;
+    const int *list;          // blocks to compute (NULL: every block)
+    const int *list_n;        // number of entries of list (device)
 };
 
 constexpr int FWD_BX = 2, FWD_BY = 4, FWD_THREADS = 512;
@@ -80,7 +84,9 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
         *reinterpret_cast<uint4 *>(wl + row * WS + c8 * 16) = reinterpret_cast<const uint4 *>(a.w)[i];
     }
     const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
-    const int total = nbx * nby * nbz;
+    // blocks to compute: the list's entries (a region of the volume) or all of them
+    const int total = a.list ? *a.list_n : nbx * nby * nbz;
+    auto block_of = [&](int i) { return a.list ? a.list[i] : i; };
     const int wave = tid >> 6, lane = tid & 63;
     const int wx = wave / FWD_BY, wy = wave % FWD_BY;
     const int m = lane & 31, kh = lane >> 5;
@@ -106,8 +112,9 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
                 pf[u] = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
         }
     };
-    if (blockIdx.x < total) fetch(blockIdx.x);
-    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    if ((int)blockIdx.x < total) fetch(block_of(blockIdx.x));
+    for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
+        const int b = block_of(bi);
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
         // LDS-only barriers: the previous block's output stores stay in flight (a
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) rs[i] = a.resid[(vrow + mfma_row(i, lane)) * 32 + m];
         }
-        if (b + (int)gridDim.x < total) fetch(b + gridDim.x);
+        if (bi + (int)gridDim.x < total) fetch(block_of(bi + gridDim.x));
         if (!live) continue;
         // two independent accumulation chains (by tap parity), so consecutive MFMAs
         // of a wave do not wait for each other's results
@@ -410,15 +417,18 @@ __global__ __launch_bounds__(256) void cnn_pack(const float *__restrict__ a, con
 
 __global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, const float *__restrict__ r,
                                                  float bound, const float *__restrict__ dmax, float region_frac,
-                                                 float *__restrict__ out, unsigned long long *__restrict__ counts,
-                                                 long long n) {
+                                                 float apply_frac, float *__restrict__ out,
+                                                 unsigned long long *__restrict__ counts, long long n) {
     __shared__ unsigned red[2][256];
     const float region_min = region_frac * *dmax;
+    const float apply_min = apply_frac * *dmax;
     unsigned ex = 0, reg = 0;
     for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
-        const float rv = r[v], d = dp[v];
+        const float d = dp[v];
+        const bool applied = d >= apply_min;     // r is read (and defined) only there
+        const float rv = applied ? r[v] : 0.0f;
         const bool ok = fabsf(rv) <= bound;      // NaN fails too
-        out[v] = ok ? d * (1.0f + rv) : d;
+        out[v] = (ok && applied) ? d * (1.0f + rv) : d;
         const bool in_region = d >= region_min;
         ex += (!ok && in_region) ? 1u : 0u;
         reg += in_region ? 1u : 0u;
@@ -436,6 +446,39 @@ __global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, c
     if (threadIdx.x == 0) {
         if (red[0][0]) atomicAdd(&counts[0], (unsigned long long)red[0][0]);
         if (red[1][0]) atomicAdd(&counts[1], (unsigned long long)red[1][0]);
+    }
+}
+
+// Region blocks of the forward kernel (FWD_BX x FWD_BY x 32 output voxels):
+// flag[b] = 1 if block b holds a voxel with dp >= frac * max; count reset.
+__global__ __launch_bounds__(256) void cnn_block_flags(const float *__restrict__ dp, int n0, int n1, int n2,
+                                                       const float *__restrict__ dmax, float frac,
+                                                       int *__restrict__ flag, int *__restrict__ count) {
+    const int nby = (n1 + FWD_BY - 1) / FWD_BY, nbz = n2 >> 5;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const float thr = frac * *dmax;
+    const int t = threadIdx.x;                       // 256 = FWD_BX * FWD_BY * 32 voxels
+    const int z = bz * 32 + (t & 31), y = by * FWD_BY + ((t >> 5) % FWD_BY), x = bx * FWD_BX + (t >> 5) / FWD_BY;
+    const bool hit = x < n0 && y < n1 && dp[((long long)x * n1 + y) * n2 + z] >= thr;
+    const int any = __syncthreads_or(hit ? 1 : 0);
+    if (t == 0) flag[b] = any ? 1 : 0;
+    if (b == 0 && t == 0) *count = 0;
+}
+
+// list the blocks within (3, 2, 1) blocks (6 voxels, the network's receptive field)
+// of a flagged block; the order follows the atomic slots (the outputs do not depend on it)
+__global__ __launch_bounds__(256) void cnn_block_list(const int *__restrict__ flag, int nbx, int nby, int nbz,
+                                                      int *__restrict__ list, int *__restrict__ count) {
+    const int total = nbx * nby * nbz;
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < total; b += gridDim.x * 256) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        bool near = false;
+        for (int i = max(bx - 3, 0); i <= min(bx + 3, nbx - 1) && !near; ++i)
+            for (int j = max(by - 2, 0); j <= min(by + 2, nby - 1) && !near; ++j)
+                for (int k = max(bz - 1, 0); k <= min(bz + 1, nbz - 1); ++k)
+                    if (flag[(i * nby + j) * nbz + k]) { near = true; break; }
+        if (near) list[atomicAdd(count, 1)] = b;
     }
 }
 
@@ -496,7 +539,9 @@ int wgrad_blocks(int n0, int n1, int n2) {
 extern "C" {
 
 int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, int epi, const void *resid, void *y,
-                     void *act, float *y32, int n0, int n1, int n2, void *stream) {
+                     void *act, float *y32, int n0, int n1, int n2, const int *blocks, const int *nblocks,
+                     void *stream) {
+    if ((blocks == nullptr) != (nblocks == nullptr)) return pvd_set_error(PVD_EINVAL, "blocks and nblocks go together");
     if (int rc = check_grid(n0, n1, n2)) return rc;
     if (!x || !w) return pvd_set_error(PVD_EINVAL, "null input or weights");
     if (cin != 16 && cin != 32) return pvd_set_error(PVD_EINVAL, "cin must be 16 or 32 (got %d)", cin);
@@ -514,24 +559,24 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
         ((reinterpret_cast<uintptr_t>(y) | reinterpret_cast<uintptr_t>(act)) & 15))
         return pvd_set_error(PVD_EINVAL, "y and act must be 16-byte aligned");
     ConvArgs a{(const uint16_t *)x, (const uint16_t *)w, bias, (const uint16_t *)resid, (uint16_t *)y,
-               (uint16_t *)act, y32, n0, n1, n2};
+               (uint16_t *)act, y32, n0, n1, n2, blocks, nblocks};
     const long long total = (long long)((n0 + FWD_BX - 1) / FWD_BX) * ((n1 + FWD_BY - 1) / FWD_BY) * (n2 >> 5);
-    const int blocks = (int)(total < num_cus() ? total : num_cus());
+    const int grid = (int)(total < num_cus() ? total : num_cus());
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     if (cin == 16) {
         switch (epi) {
-            case 0: e = launch_fwd<16, 0>(a, blocks, st); break;
-            case 1: e = launch_fwd<16, 1>(a, blocks, st); break;
-            case 2: e = launch_fwd<16, 2>(a, blocks, st); break;
-            default: e = launch_fwd<16, 3>(a, blocks, st); break;
+            case 0: e = launch_fwd<16, 0>(a, grid, st); break;
+            case 1: e = launch_fwd<16, 1>(a, grid, st); break;
+            case 2: e = launch_fwd<16, 2>(a, grid, st); break;
+            default: e = launch_fwd<16, 3>(a, grid, st); break;
         }
     } else {
         switch (epi) {
-            case 0: e = launch_fwd<32, 0>(a, blocks, st); break;
-            case 1: e = launch_fwd<32, 1>(a, blocks, st); break;
-            case 2: e = launch_fwd<32, 2>(a, blocks, st); break;
-            default: e = launch_fwd<32, 3>(a, blocks, st); break;
+            case 0: e = launch_fwd<32, 0>(a, grid, st); break;
+            case 1: e = launch_fwd<32, 1>(a, grid, st); break;
+            case 2: e = launch_fwd<32, 2>(a, grid, st); break;
+            default: e = launch_fwd<32, 3>(a, grid, st); break;
         }
     }
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch CNN convolution");
@@ -611,15 +656,40 @@ int pvd_cnn_pack_inputs(const float *activity, const float *density, const float
 }
 
 int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const float *dose_max, float region_frac,
-                             float *out, unsigned long long *counts, long long n, void *stream) {
-    if (!dp || !r || !dose_max || !out || !counts || n < 1 || !(bound >= 0.0f) || !(region_frac >= 0.0f))
+                             float apply_frac, float *out, unsigned long long *counts, long long n, void *stream) {
+    if (!dp || !r || !dose_max || !out || !counts || n < 1 || !(bound >= 0.0f) || !(region_frac >= 0.0f) ||
+        !(apply_frac >= 0.0f))
         return pvd_set_error(PVD_EINVAL, "bad argument");
     long long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(cnn_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, r, bound, dose_max,
-                       region_frac, out, counts, n);
+                       region_frac, apply_frac, out, counts, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch correction");
+}
+
+int pvd_cnn_region_blocks(const float *dp, int n0, int n1, int n2, const float *dose_max, float frac, int *flags,
+                          int *blocks, int *nblocks, void *stream) {
+    if (int rc = check_grid(n0, n1, n2)) return rc;
+    if (!dp || !dose_max || !flags || !blocks || !nblocks || !(frac >= 0.0f))
+        return pvd_set_error(PVD_EINVAL, "bad argument");
+    const int nbx = (n0 + FWD_BX - 1) / FWD_BX, nby = (n1 + FWD_BY - 1) / FWD_BY, nbz = n2 >> 5;
+    const long long total = (long long)nbx * nby * nbz;
+    if (total > 0x7fffffffLL) return pvd_set_error(PVD_ENOTSUP, "too many blocks");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(cnn_block_flags, dim3((unsigned)total), dim3(256), 0, st, dp, n0, n1, n2, dose_max, frac, flags,
+                       nblocks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pvd_hip_error(e, "launch region flags");
+    const unsigned g = (unsigned)std::min<long long>((total + 255) / 256, 1024);
+    hipLaunchKernelGGL(cnn_block_list, dim3(g), dim3(256), 0, st, (const int *)flags, nbx, nby, nbz, blocks, nblocks);
+    e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch region list");
+}
+
+size_t pvd_cnn_region_block_count(int n0, int n1, int n2) {
+    if (n0 < 1 || n1 < 1 || n2 < 32) return 0;
+    return (size_t)((n0 + FWD_BX - 1) / FWD_BX) * ((n1 + FWD_BY - 1) / FWD_BY) * (n2 >> 5);
 }
 
 int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream) {

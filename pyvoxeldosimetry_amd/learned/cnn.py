@@ -120,9 +120,10 @@ class ResidualCNN:
             n = shape[0] * shape[1] * shape[2]
             bf = dict(dtype=torch.bfloat16, device=self.device)
             ws = {
-                "h": [torch.empty((n, 32), **bf) for _ in range(5)],
+                # zeroed once: a region-restricted forward reads neighbours it did not write
+                "h": [torch.zeros((n, 32), **bf) for _ in range(5)],
                 "act": [None] + [torch.empty((n, 32), **bf) for _ in range(4)],
-                "r": torch.empty(n, dtype=torch.float32, device=self.device),
+                "r": torch.zeros(n, dtype=torch.float32, device=self.device),
                 "dz": torch.empty((n, 32), **bf),
                 "dr": torch.empty((n, 32), **bf),
                 "dh": torch.empty((n, 32), dtype=torch.float32, device=self.device),
@@ -138,23 +139,28 @@ class ResidualCNN:
     def _stream(self):
         return ctypes.c_void_p(nat.current_stream_handle(self.device))
 
-    def _conv(self, x, cin, l, epi, shape, y=None, act=None, resid=None, y32=None, weights=None, bias=True):
+    def _conv(self, x, cin, l, epi, shape, y=None, act=None, resid=None, y32=None, weights=None, bias=True,
+              blocks=None):
+        bl, bn = blocks if blocks is not None else (None, None)
         nat.check(self.lib.pvd_cnn_conv_fwd(_p(x), cin, _p(weights if weights is not None else self.w_bf16(l)),
                                             _p(self.bias(l)) if bias else ctypes.c_void_p(0), epi, _p(resid),
-                                            _p(y), _p(act), _p(y32), shape[0], shape[1], shape[2], self._stream()),
+                                            _p(y), _p(act), _p(y32), shape[0], shape[1], shape[2], _p(bl), _p(bn),
+                                            self._stream()),
                   "pvd_cnn_conv_fwd")
 
     # ------------------------------------------------------------------ forward / backward
-    def forward(self, x, shape, keep_activations: bool = True):
+    def forward(self, x, shape, keep_activations: bool = True, blocks=None):
         """x: BF16 [n, 16] inputs; returns the FP32 correction r [n] (workspace tensor).
-        ``keep_activations=False`` (inference) skips the ReLU outputs backward needs."""
+        ``keep_activations=False`` (inference) skips the ReLU outputs backward needs.
+        ``blocks``: (block list, count) device tensors from ``pvd_cnn_region_blocks``:
+        every layer runs on those blocks only, and r is defined on the region they cover."""
         ws = self._workspace(shape)
         h = ws["h"]
-        self._conv(x, 16, 0, nat.PVD_CNN_EPI_RELU, shape, y=h[0])
+        self._conv(x, 16, 0, nat.PVD_CNN_EPI_RELU, shape, y=h[0], blocks=blocks)
         for l in range(1, 5):
             self._conv(h[l - 1], 32, l, nat.PVD_CNN_EPI_RESID_RELU, shape, y=h[l],
-                       act=ws["act"][l] if keep_activations else None, resid=h[l - 1])
-        self._conv(h[4], 32, 5, nat.PVD_CNN_EPI_LINEAR_F32, shape, y32=ws["r"])
+                       act=ws["act"][l] if keep_activations else None, resid=h[l - 1], blocks=blocks)
+        self._conv(h[4], 32, 5, nat.PVD_CNN_EPI_LINEAR_F32, shape, y32=ws["r"], blocks=blocks)
         return ws["r"]
 
     def loss_and_grad(self, dp, dt, mask, shape):

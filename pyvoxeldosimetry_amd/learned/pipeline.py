@@ -4,7 +4,9 @@ plain dose     D_p = conv(A, K_mc) * rho_w / rho      (HIP FFT path, linear mode
                K_mc: Monte Carlo point kernel of the nuclide in water (same
                transport model as the targets, so water is consistent)
 correction     r = CNN(A / max A, rho, D_p / max D_p)  (BF16 MFMA kernels)
-corrected      D = D_p (1 + r) where |r| <= bound, D_p elsewhere; if more than
+corrected      D = D_p (1 + r) where |r| <= bound, D_p elsewhere (optionally only where
+               D_p >= region_frac * max D_p, the network then running only on the
+               blocks within its receptive field of that region).  If more than
                `max_exceed_frac` of the >= 10 %-max voxels exceed the bound, the whole
                volume falls back to D_p (reported in the result)
 targets        Monte Carlo dose of the same activity in the heterogeneous medium
@@ -76,18 +78,26 @@ class Correction:
 class LearnedCorrector:
     def __init__(self, nuclide: str = "Y90", voxel_mm: float = 2.0, kernel_size: int = 13,
                  kernel_histories: int = 1 << 23, bound: float = 0.5, max_exceed_frac: float = 0.05, seed: int = 0,
-                 device=None):
+                 device=None, region_frac: float = 0.05, photons: bool = True):
+        """``photons``: the Monte Carlo (point kernel and targets) includes the nuclide's
+        photons (gamma lines, bremsstrahlung); weights record the setting they were
+        trained with."""
         import torch
         nat.require_device()
         self.lib = nat.load()
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.voxel_mm = float(voxel_mm)
-        self.mc = MonteCarlo(nuclide, device=self.device)
+        self.nuclide = nuclide
+        self.photons = bool(photons)
+        self.mc = MonteCarlo(nuclide, device=self.device, photons=self.photons)
         k = self.mc.point_kernel(voxel_mm, kernel_size, kernel_histories)
         self.kernel = torch.from_numpy(k).to(self.device)
         self.net = ResidualCNN(seed=seed, device=self.device)
         self.bound = float(bound)
         self.max_exceed_frac = float(max_exceed_frac)
+        # the network corrects voxels with plain dose >= region_frac * max (below the 10 %
+        # evaluation threshold, so that region is covered); the plain dose stands elsewhere
+        self.region_frac = float(region_frac)
         self._plans = {}
         self._bufs = {}
 
@@ -99,10 +109,15 @@ class LearnedCorrector:
         if name is None:
             raise ValueError(f"no pretrained weights for {nuclide} at {voxel_mm} mm; "
                              f"available: {sorted(PRETRAINED)}")
-        meta = dict(bound=0.9)
+        path = os.path.join(WEIGHTS_DIR, name)
+        from safetensors import safe_open
+        with safe_open(path, framework="pt") as f:
+            file_meta = dict(f.metadata() or {})
+        # the Monte Carlo the weights were trained against (files without the key: electrons only)
+        meta = dict(bound=0.9, photons=file_meta.get("mc_photons", "0") == "1")
         meta.update(kw)
         c = cls(nuclide, voxel_mm=voxel_mm, **meta)
-        c.weights_meta = c.net.load_weights(os.path.join(WEIGHTS_DIR, name))
+        c.weights_meta = c.net.load_weights(path)
         return c
 
     def _stream(self):
@@ -163,17 +178,41 @@ class LearnedCorrector:
         self.net.adam_step(lr)
         return sums
 
-    def correct(self, act, rho, r_override=None) -> Correction:
+    def region_blocks(self, dp, dmax, frac: float):
+        """Device list of the network blocks within its receptive field of dp >= frac * max."""
+        import torch
+        shape = tuple(dp.shape)
+        nb = int(self.lib.pvd_cnn_region_block_count(*shape))
+        key = (shape, "region")
+        if key not in self._bufs:
+            self._bufs[key] = (torch.empty(nb, dtype=torch.int32, device=self.device),
+                               torch.empty(nb, dtype=torch.int32, device=self.device),
+                               torch.empty(1, dtype=torch.int32, device=self.device))
+        flags, blist, count = self._bufs[key]
+        nat.check(self.lib.pvd_cnn_region_blocks(ctypes.c_void_p(dp.data_ptr()), *shape, ctypes.c_void_p(dmax.data_ptr()),
+                                                 ctypes.c_float(frac), ctypes.c_void_p(flags.data_ptr()),
+                                                 ctypes.c_void_p(blist.data_ptr()), ctypes.c_void_p(count.data_ptr()),
+                                                 self._stream()), "pvd_cnn_region_blocks")
+        return blist, count
+
+    def correct(self, act, rho, r_override=None, region: bool = False) -> Correction:
         """CNN-corrected dose with fallback (one host synchronisation, for the
-        fallback decision).  `r_override` (tests) replaces the network output,
-        e.g. to force an out-of-bound correction.  `plain` and `r` of the result
-        are views of buffers the next call on this shape overwrites."""
+        fallback decision).  With ``region`` the network runs only on the blocks within
+        its receptive field of the region dp >= region_frac * max dp, and the correction
+        applies there (the plain dose elsewhere; exact on that region).  The default
+        runs it on the whole volume: the >= 10 %-max region of the Monte Carlo dose is
+        not known before the correction, and in lung the plain dose's maximum can be
+        several times the Monte Carlo one, so a plain-dose threshold can miss part of it.  `r_override` (tests) replaces the network output,
+        e.g. to force an out-of-bound correction.  `plain` and `r` of the result are
+        views of buffers the next call on this shape overwrites."""
         import torch
         shape = tuple(act.shape)
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
+        apply_frac = self.region_frac if region else 0.0
         if r_override is None:
             x, dmax = self.pack(act, rho, dp)
-            r = self.net.forward(x, shape, keep_activations=False)
+            blocks = self.region_blocks(dp, dmax, apply_frac) if region else None
+            r = self.net.forward(x, shape, keep_activations=False, blocks=blocks)
         else:
             dmax = dp.max().reshape(1)
             r = r_override.reshape(-1).contiguous()
@@ -181,7 +220,8 @@ class LearnedCorrector:
         counts = torch.zeros(2, dtype=torch.int64, device=self.device)
         nat.check(self.lib.pvd_cnn_apply_correction(ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(r.data_ptr()),
                                                     ctypes.c_float(self.bound), ctypes.c_void_p(dmax.data_ptr()),
-                                                    ctypes.c_float(0.1), ctypes.c_void_p(out.data_ptr()),
+                                                    ctypes.c_float(0.1), ctypes.c_float(apply_frac),
+                                                    ctypes.c_void_p(out.data_ptr()),
                                                     ctypes.c_void_p(counts.data_ptr()), dp.numel(), self._stream()),
                   "pvd_cnn_apply_correction")
         n_ex, n_region = (int(v) for v in counts.tolist())

@@ -93,17 +93,24 @@ def main(argv=None):
     ap.add_argument("--save", default="gpurun_out/cnn_y90_2mm.pt", help="weights + Adam state (torch.save)")
     ap.add_argument("--export", default=None, help="also write the FP32 weights as safetensors")
     ap.add_argument("--resume", default=None, help="continue from a --save file (weights and Adam state)")
+    ap.add_argument("--init", default=None, help="start from these FP32 weights (safetensors), fresh Adam state")
     ap.add_argument("--seed0", type=int, default=TRAIN_SEED0, help="first training phantom seed")
+    ap.add_argument("--nuclide", default="Y90", choices=["Y90", "Lu177"])
+    ap.add_argument("--voxel", type=float, default=2.0, help="voxel size in mm")
+    ap.add_argument("--no-photons", action="store_true", help="electrons-only Monte Carlo")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
     shape = (args.shape,) * 3
-    corr = LearnedCorrector("Y90", voxel_mm=2.0, bound=args.bound)
+    corr = LearnedCorrector(args.nuclide, voxel_mm=args.voxel, bound=args.bound, photons=not args.no_photons)
     if args.resume:
         corr.net.load_state_dict(torch.load(args.resume, weights_only=True))
         log(f"resumed from {args.resume} at Adam step {corr.net.step}")
+    elif args.init:
+        corr.net.load_weights(args.init)
+        log(f"initialised from {args.init}")
     if args.seed0 + args.n_train > EVAL_SEED0 - 8:
         raise SystemExit("training seeds would overlap the validation/evaluation seeds")
     log(f"building {args.n_train} training triples {shape} from seed {args.seed0}")
@@ -137,10 +144,12 @@ def main(argv=None):
     tm = time_per_volume(corr, shape)
     torch.save(corr.net.state_dict(), args.save)
     if args.export:
-        corr.net.save_weights(args.export, {"nuclide": "Y90", "voxel_mm": 2.0, "shape": args.shape,
+        corr.net.save_weights(args.export, {"nuclide": args.nuclide, "voxel_mm": args.voxel, "shape": args.shape,
+                                            "mc_photons": "0" if args.no_photons else "1",
                                             "command": "python -m pyvoxeldosimetry_amd.learned.train "
                                                        + " ".join(sys.argv[1:] if argv is None else argv),
                                             "train_seeds": f"{args.seed0}..{args.seed0 + args.n_train - 1}",
+                                            "init": args.init or "",
                                             "train_histories": args.train_histories, "epochs": args.epochs,
                                             "adam_steps": corr.net.step, "eval": json.dumps(ev)})
     rep = {"n_train": args.n_train, "n_eval": args.n_eval, "shape": list(shape), "epochs": args.epochs,

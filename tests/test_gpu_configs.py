@@ -136,15 +136,38 @@ def test_cnn_forward_128cubed_matches_the_network_oracle_on_windows(cnn128, star
     assert diff <= 2e-2 and fro <= 5e-3
 
 
-def test_corrected_dose_128cubed_applies_the_bounded_correction(cnn128):
+@pytest.mark.parametrize("region", [True, False])
+def test_corrected_dose_128cubed_applies_the_bounded_correction(cnn128, region):
     import torch
     corr, act, rho = cnn128
-    _say("cnn 128^3 corrected dose")
-    c = corr.correct(act, rho)
+    _say("cnn 128^3 corrected dose", region)
+    c = corr.correct(act, rho, region=region)
     assert not c.fallback
-    assert torch.isfinite(c.r).all() and torch.isfinite(c.dose).all()
-    ok = c.r.abs() <= corr.bound
+    applied = c.plain >= (corr.region_frac if region else 0.0) * c.plain.max()
+    assert torch.isfinite(c.r[applied]).all() and torch.isfinite(c.dose).all()
+    ok = (c.r.abs() <= corr.bound) & applied
     expected = torch.where(ok, c.plain * (1.0 + c.r), c.plain)
     torch.testing.assert_close(c.dose, expected, rtol=2e-6, atol=0)
-    region = c.plain >= 0.1 * c.plain.max()
-    assert c.exceed == int((~ok & region).sum().item())
+    ten = c.plain >= 0.1 * c.plain.max()
+    assert c.exceed == int((~(c.r.abs() <= corr.bound) & ten).sum().item())
+
+
+@pytest.mark.parametrize("seed", [20000, 20002])
+def test_region_restricted_network_equals_the_full_network_on_its_region(seed):
+    """The network run only on the blocks within its 6-voxel receptive field of the
+    region dp >= 5 % of max gives the full-volume network's output bit for bit there;
+    outside, the dose is the plain dose.  Seed 20000 has a small region (~1 % of the
+    volume), 20002 a large one (~94 %)."""
+    import torch
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, synth_phantom
+    _say("cnn region vs full", seed)
+    corr = LearnedCorrector.pretrained("Y90", 2.0)
+    act, rho = synth_phantom(seed, (128, 128, 128), corr.device)
+    full = corr.correct(act, rho, region=False)
+    full_dose, full_r = full.dose.clone(), full.r.clone()
+    part = corr.correct(act, rho, region=True)
+    reg = part.plain >= corr.region_frac * part.plain.max()
+    assert reg.any()
+    assert torch.equal(part.r[reg], full_r[reg])
+    assert torch.equal(part.dose[reg], full_dose[reg])
+    assert torch.equal(part.dose[~reg], part.plain[~reg])

@@ -40,7 +40,7 @@ def _conv(lib, x, cin, w, bias, epi, shape, y=None, act=None, resid=None, y32=No
     import torch
     from pyvoxeldosimetry_amd import _native as nat
     p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)  # noqa: E731
-    nat.check(lib.pvd_cnn_conv_fwd(p(x), cin, p(w), p(bias), epi, p(resid), p(y), p(act), p(y32), *shape,
+    nat.check(lib.pvd_cnn_conv_fwd(p(x), cin, p(w), p(bias), epi, p(resid), p(y), p(act), p(y32), *shape, None, None,
                                    ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "conv")
 
 
@@ -380,15 +380,15 @@ def test_correction_fallback_rule():
     act, rho = synth_phantom(5, (32, 32, 32), corr.device)
     dp = corr.plain(act, rho).clone()
     r = torch.full(act.shape, 0.1, device="cuda")
-    c = corr.correct(act, rho, r_override=r)
+    c = corr.correct(act, rho, r_override=r, region=False)
     assert not c.fallback and c.exceed == 0
     torch.testing.assert_close(c.dose, dp * 1.1, rtol=1e-6, atol=0)
     r_bad = r.clone()
     r_bad[0, 0, 0] = 7.0                                   # one voxel out of bound: voxelwise fallback
-    c = corr.correct(act, rho, r_override=r_bad)
+    c = corr.correct(act, rho, r_override=r_bad, region=False)
     assert c.dose[0, 0, 0].item() == pytest.approx(dp[0, 0, 0].item())
     r_bad = torch.full(act.shape, -3.0, device="cuda")     # everything out of bound: whole-volume fallback
-    c = corr.correct(act, rho, r_override=r_bad)
+    c = corr.correct(act, rho, r_override=r_bad, region=False)
     assert c.fallback and c.exceed > 0
     torch.testing.assert_close(c.dose, dp)
 
@@ -408,7 +408,7 @@ def test_short_training_fits_its_phantoms():
     def mean_err(use_cnn):
         e = []
         for act, rho, tgt in data[:6]:
-            c = corr.correct(act, rho)
+            c = corr.correct(act, rho, region=False)
             e.append(voxel_errors(c.dose if use_cnn else c.plain, tgt).mean().item())
         return sum(e) / len(e)
 
