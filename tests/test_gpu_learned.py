@@ -450,8 +450,9 @@ def test_gpu_training_from_scratch_beats_plain_on_held_out_monte_carlo():
 
 
 def test_pretrained_correction_beats_plain_on_held_out_monte_carlo():
-    """The shipped weights (learned/weights, trained on seeds 10000..10299) against
-    Monte Carlo of held-out phantoms (seeds 20000..): the corrected dose is much
+    """The shipped weights (learned/weights: trained at 64^3 on seeds 10000..11199, then
+    at 128^3 on seeds 10000..11999) against Monte Carlo of held-out phantoms (seeds
+    20000..): the corrected dose is much
     closer to Monte Carlo than the plain density-scaled convolution."""
     from pyvoxeldosimetry_amd.learned.pipeline import EVAL_SEEDS_TEST, LearnedCorrector, evaluate
     _say("pretrained vs held-out MC")
