@@ -748,6 +748,49 @@ __device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Fused register two-step passes (P = 16 B), one LDS exchange per transform:
+//   step 1 straight from HBM: item (b, l) loads x[b + B n1] (n1 < 16) of column l,
+//     DFT-16, twiddles W_P^(b k1), writes LDS row b + B k1 (= k1 B + b ordering of
+//     xy_nat2perm after its first step);
+//   step 2 in registers: item (k1, l) reads rows k1 B .. k1 B + B-1 and does the
+//     DFT-B: X[k1 + 16 k2] in v[k2].
+// FY stores X straight to HBM (row k1 + 16 k2).  X multiplies by the kernel spectrum
+// in the same registers, runs the inverse DFT-B and twiddles W_P^(-b k1) (the first
+// step of xy_perm2nat), writes the rows back, and after one barrier the inverse DFT-16
+// of item (b, l) leaves x[b + B n1] in registers.  Outputs with no reference-mode fold
+// go straight to HBM (row o reads index (o + off) mod P); with a fold the tile is
+// written back to LDS and stored by the row maps of store_col_tile(_perm).
+
+// step 1 from HBM (all loads of an item issued before any is used; rows >= rows_in and
+// columns >= nc are zero); entered after a barrier that published tw
+template <int B, bool INV>
+__device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl, const float2 *__restrict__ tw,
+                                                const float2 *__restrict__ g, long long pitch, int rows_in, int nc) {
+    const int TK = 1 << lnl;
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = g[(long long)min(b + B * n1, rows_in - 1) * pitch + l];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1)
+            if (b + B * n1 >= rows_in || l >= nc) v[n1] = make_float2(0.f, 0.f);
+        Dft<16, INV>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+            float2 t = tw[b * k1];
+            if (INV) t.y = -t.y;
+            v[k1] = cmul(v[k1], t);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) s[(b + B * k1) * TK + l] = v[k1];
+    }
+}
+
+// output row o of FFT index j: (j - off) mod P
+__device__ __forceinline__ int out_row(int j, int off, int P) { return j >= off ? j - off : j + P - off; }
+
 template <int B, bool INV>
 __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const pvd_axis_d &ay = v.ax[1];
@@ -759,11 +802,34 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nc = min(TK, H - c0);
     float2 *g = v.work + (long long)x * v.wplane + c0;
     load_twiddles(tw, ay);
-    load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
     lds_barrier();
-    xy_nat2perm<B, INV>(s, lnl, tw);
-    if (!INV) store_col_tile_perm<B>(g, Hp, s, Py, lnl, Py, 0, 0, nc);
-    else store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
+    xy_step1_global<B, INV>(s, lnl, tw, g, Hp, INV ? Py : Ny, nc);
+    lds_barrier();
+    const bool direct = !INV || ay.foldn == 0;     // block-uniform
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        float2 u[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) u[n2] = s[(k1 * B + n2) * TK + l];
+        dft_any<B, INV>(u);
+        if (direct) {
+            if (l < nc) {
+#pragma unroll
+                for (int k2 = 0; k2 < B; ++k2) {
+                    const int j = k1 + 16 * k2;
+                    const int o = INV ? out_row(j, ay.off, Py) : j;
+                    if (o < (INV ? Ny : Py)) g[(long long)o * Hp + l] = u[k2];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = u[k2];
+        }
+    }
+    if (!direct) {
+        lds_barrier();
+        store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
+    }
 }
 
 template <int B>
@@ -773,65 +839,82 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int lnl = v.ltkx, TK = 1 << lnl;
     const int ntc = (H + TK - 1) >> lnl;
     const int u0 = tile / ntc;
-    const int y = x_tile_row(u0, v.ax[1].n);
+    const int Py = v.ax[1].n;
+    const int y = x_tile_row(u0, Py);
     const int c0 = (tile - u0 * ntc) << lnl;
     const int nc = min(TK, H - c0);
     const long long plane = v.wplane;                 // work
-    const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
+    const long long splane = (long long)Py * Hp;      // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
     const float2 *sp = v.spec + off0;
-    // After the forward two-step, LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2.
-    // Every spectrum load is issued at a clamped address (the tile's first entry where
-    // masked) and masked afterwards.  (Loading the first batch before the forward
-    // transform took the 35-point instance from 131 to 170 VGPRs and X from 471 to
-    // 735 us on the 512x512x400 torso: it halved the workgroups per CU.)
-    const int total = Px << lnl;
     const float *sr = reinterpret_cast<const float *>(v.spec);
-    const long long pyh = v.ax[1].n / 2 + 1;
-    const long long row0 = (long long)min(y, v.ax[1].n - y) * Hp + c0;
-    auto sym_index = [&](int e) -> long long {
-        const int r = e >> lnl, l = e & (TK - 1);
-        const int k = r / B + 16 * (r % B);
-        return (e < total && l < nc) ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0;
-    };
+    const long long pyh = Py / 2 + 1;
+    const long long row0 = (long long)min(y, Py - y) * Hp + c0;
     load_twiddles(tw, ax);
-    load_col_tile(s, g, plane, Px, Nx, lnl, nc);
     lds_barrier();
-    xy_nat2perm<B, false>(s, lnl, tw);
-    if (v.sym) {
-        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
-            float sv[kBatch];
+    xy_step1_global<B, false>(s, lnl, tw, g, plane, Nx, nc);
+    lds_barrier();
+    // forward DFT-B, times the spectrum, inverse DFT-B and twiddles, in registers
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        // spectrum of the frequencies k1 + 16 k2 (issued before the transform; masked
+        // columns read the tile's first entry and are never stored)
+        const bool ok = l < nc;
+        float2 sv[B];
+        if (v.sym) {
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) sv[u] = sr[sym_index(e0 + u * blockDim.x)];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * blockDim.x;
-                const bool ok = e < total && (e & (TK - 1)) < nc;
-                if (e < total) s[e] = cscale(s[e], ok ? sv[u] : 0.f);
+            for (int k2 = 0; k2 < B; ++k2) {
+                const int k = k1 + 16 * k2;
+                sv[k2] = make_float2(sr[ok ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0], 0.f);
             }
+        } else {
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) sv[k2] = sp[ok ? (long long)(k1 + 16 * k2) * splane + l : 0];
         }
-    } else {
-        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
-            float2 sv[kBatch];
+        float2 u[B];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * blockDim.x;
-                const int r = e >> lnl, l = e & (TK - 1);
-                const int k = r / B + 16 * (r % B);
-                sv[u] = sp[(e < total && l < nc) ? k * splane + l : 0];
-            }
+        for (int n2 = 0; n2 < B; ++n2) u[n2] = s[(k1 * B + n2) * TK + l];
+        dft_any<B, false>(u);
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = e0 + u * blockDim.x;
-                const bool ok = e < total && (e & (TK - 1)) < nc;
-                if (e < total) s[e] = cmul(s[e], ok ? sv[u] : make_float2(0.f, 0.f));
-            }
+        for (int k2 = 0; k2 < B; ++k2) u[k2] = v.sym ? cscale(u[k2], sv[k2].x) : cmul(u[k2], sv[k2]);
+        dft_any<B, true>(u);
+#pragma unroll
+        for (int b = 1; b < B; ++b) {
+            float2 t = tw[b * k1];
+            t.y = -t.y;
+            u[b] = cmul(u[b], t);
         }
+#pragma unroll
+        for (int b = 0; b < B; ++b) s[(k1 * B + b) * TK + l] = u[b];
     }
     lds_barrier();
-    xy_perm2nat<B, true>(s, lnl, tw);
-    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
+    // inverse DFT-16 of item (b, l): x[b + B n1] in registers
+    const bool direct = ax.foldn == 0;               // block-uniform
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 u[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) u[k1] = s[(k1 * B + b) * TK + l];
+        Dft<16, true>::run(u);
+        if (direct) {
+            if (l < nc) {
+#pragma unroll
+                for (int n1 = 0; n1 < 16; ++n1) {
+                    const int o = out_row(b + B * n1, ax.off, Px);
+                    if (o < Nx) g[(long long)o * plane + l] = u[n1];
+                }
+            }
+        } else {
+            // rows b + B n1 are read only by this item: write back in natural order
+#pragma unroll
+            for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = u[n1];
+        }
+    }
+    if (!direct) {
+        lds_barrier();
+        store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -1006,14 +1089,13 @@ __device__ __forceinline__ void run_batch(const pvd_batch_args &a) {
 }
 
 // The y/x register two-step for P = 16 * B has its own kernels, so that their
-// register budget does not touch the other instances.  Four waves per SIMD (at most
-// 128 VGPRs) took the 560-point FY/IY from 273 to 178 us on the 512x512x400 torso;
-// the 28, 30 and 32-point X passes would spill at 128 and keep three.
-template <int B> constexpr int xy_waves() { return (B == 28 || B == 30 || B == 32) ? 3 : 4; }
-// the batch X instances of B = 35, 36 need the descriptor lookup's registers too and
-// would spill at 128
-template <int PASS, int B> constexpr int xy_waves_batch() {
-    return (PASS == PVD_PASS_X && B >= 35) ? 3 : xy_waves<B>();
+// register budget does not touch the other instances.  Waves per SIMD (exactly; the
+// VGPR cap is 512 / waves): FY/IY hold 16 loads or a B-point line per item and keep
+// four up to B = 27; X also holds the B spectrum values of its line through the
+// forward transform and keeps four up to B = 21, three up to 30 and two above (a
+// 512-thread block of the 512-point X tile is one block per CU either way).
+template <int PASS, int B> constexpr int xy_waves() {
+    return PASS == PVD_PASS_X ? (B <= 21 ? 4 : B <= 30 ? 3 : 2) : (B <= 27 ? 4 : 3);
 }
 }  // namespace
 
@@ -1038,7 +1120,7 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch
 }
 
 template <int PASS, int B>
-__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<B>())))
+__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B>(), xy_waves<PASS, B>())))
 void pvd_fft_pass_xy(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
@@ -1049,7 +1131,7 @@ void pvd_fft_pass_xy(pvd_vol_d v) {
 }
 
 template <int PASS, int B>
-__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves_batch<PASS, B>())))
+__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B>(), xy_waves<PASS, B>())))
 void pvd_fft_pass_xy_batch(pvd_batch_args a) {
     run_batch<PASS, false, 16, B>(a);
 }
