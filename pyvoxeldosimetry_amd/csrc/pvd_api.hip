@@ -213,8 +213,10 @@ struct pvd_batch {
 
 namespace {
 
+// 16 * B for the two-step instances (256 among them: it runs there when the 256-point
+// path does not, with a reference-mode fold or other than 16-line tiles)
 bool xy_two_step_len(int P) {
-    if (P == 256 || P % 16) return false;
+    if (P % 16) return false;
     for (int b : kXyRegB)
         if (P == 16 * b) return true;
     return false;
@@ -226,11 +228,12 @@ int z_reg_code(int P) {
     return 0;
 }
 
-// A length whose transform runs register-resident on axis a (0 x, 1 y, 2 z): 256 on
-// y/x (16-line tiles and no reference-mode fold), 16 * B for the two-step instances on
-// y/x, and the z two-step table (16-line tiles).
+// A length whose transform runs register-resident on axis a (0 x, 1 y, 2 z): 16 * B
+// for the two-step instances on y/x (256 on the 256-point path with 16-line tiles and
+// no reference-mode fold), and the z two-step table (16-line tiles).
 bool register_len(int a, int P, bool fold, int tile) {
-    if (a < 2) return (P == 256 && !fold && tile == 16) || xy_two_step_len(P);
+    (void)fold;
+    if (a < 2) return xy_two_step_len(P);
     return tile == 16 && z_reg_code(P) != 0;
 }
 
@@ -342,28 +345,36 @@ int fft_layout(pvd_plan *p) {
     pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
     int tkx, bx; size_t ldsx;
     pick_tile(Px, p->tile[0], 0, &tkx, &bx, &ldsx);
+    const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
+    // y/x register paths: 1 = the 256-point path (16-line tiles, no reference-mode fold;
+    // checked again in the kernel); 2 = the two-step P = 16 * rb (rb in kXyRegB) in its
+    // own kernel instances, which keep 2 * foldn * TK elements of LDS past the tile for
+    // the fold (fewer lines per tile when that would pass the LDS budget)
+    for (int a = 0; a < 2; ++a) {
+        pvd_axis_d &ax = v.ax[a];
+        int &tk = a ? tky : tkx;
+        int &bl = a ? by : bx;
+        size_t &ld = a ? ldsy : ldsx;
+        const bool sq = ax.n == 256 && ax.foldn == 0 && tk == 16;
+        ax.reg = regs ? 1 : 0;
+        ax.ra = ax.ra_log = ax.rb = 0;
+        if (regs && !sq && xy_two_step_len(ax.n)) {
+            ax.reg = 2;
+            ax.ra = 16;
+            ax.ra_log = 4;
+            ax.rb = ax.n / 16;
+            auto fold = [&](int t) { return sizeof(float2) * 2 * (size_t)ax.foldn * (size_t)t; };
+            while (tk > 2 && ld + fold(tk) > kFftLdsCap) pick_tile(ax.n, tk / 2, 0, &tk, &bl, &ld);
+            ld += fold(tk);
+        }
+    }
     // room for the padded exchange array of the register-resident 256-point path
     if (Py == 256 && tky == 16) ldsy += PVD_SQ256_EXTRA_LDS;
     if (Px == 256 && tkx == 16) ldsx += PVD_SQ256_EXTRA_LDS;
     // the y and x passes move 16-byte column pairs and read whole tiles of columns
     if (tky < 2 || tkx < 2) return fail(PVD_ENOTSUP, "FFT length too large for a two-column tile");
-    // register-resident transforms: the 256-point y/x path (checked again in the
-    // kernel) and the z two-step P = A * B for 16-line tiles whose block covers
-    // 16*A threads in step 2 and 16*B items in two rounds in step 1
-    const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
-    // y/x: 1 = the 256-point path may run; 2 = the two-step P = 16 * rb (rb in
-    // kXyRegB) in its own kernel instances
-    for (int a = 0; a < 2; ++a) {
-        pvd_axis_d &ax = v.ax[a];
-        ax.reg = regs ? 1 : 0;
-        ax.ra = ax.ra_log = ax.rb = 0;
-        if (regs && xy_two_step_len(ax.n)) {
-            ax.reg = 2;
-            ax.ra = 16;
-            ax.ra_log = 4;
-            ax.rb = ax.n / 16;
-        }
-    }
+    // z two-step P = A * B for 16-line tiles whose block covers 16*A threads in step 2
+    // and 16*B items in two rounds in step 1
     pvd_axis_d &az = v.ax[2];
     az.reg = 0;
     az.ra = az.ra_log = az.rb = 0;
@@ -388,7 +399,15 @@ int fft_layout(pvd_plan *p) {
     // x-planes of the workspace are Py * Hp complex plus PVD_PLANE_PAD (0: padding by one
     // or three 128-byte lines measured the same on the CT workload)
     v.wplane = Py * v.Hp + PVD_PLANE_PAD;
-    if (Px == 256 && !x_sq256_offsets_fit(Nx, Py, v.Hp, v.wplane) && v.ax[0].reg == 1) v.ax[0].reg = 0;
+    // the 256-point X pass addresses with 32-bit offsets: larger volumes take the
+    // two-step instance for 256 (64-bit offsets; no fold, so no extra LDS)
+    if (Px == 256 && !x_sq256_offsets_fit(Nx, Py, v.Hp, v.wplane) && v.ax[0].reg == 1) {
+        v.ax[0].reg = 2;
+        v.ax[0].ra = 16;
+        v.ax[0].ra_log = 4;
+        v.ax[0].rb = 16;
+        bx = std::min(bx, PVD_XY_MAX_BLOCK);
+    }
     const long long nlines = (long long)Nx * Ny;
     const long long tz = (nlines + 2 * lz - 1) / (2 * lz);
     const long long ty = (long long)Nx * ((v.H + tky - 1) / tky);

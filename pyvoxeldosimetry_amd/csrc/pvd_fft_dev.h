@@ -646,121 +646,27 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 
 // ---------------------------------------------------------------------------------
 // Register two-step transforms of the y and x passes for lengths P = 16 * B other
-// than 256 (B = 20 ... 36), over a column tile in LDS (element (row k, line l) at
-// s[k * TK + l], TK = 2^lnl lines).  With n = B n1 + b and k = k1 + 16 k2:
+// than 256 (B = 6 ... 36), over column tiles of TK = 2^lnl lines.  With n = B n1 + b
+// and k = k1 + 16 k2:
 //   X[k1 + 16 k2] = sum_b W_B^(b k2) W_P^(b k1) sum_n1 x[B n1 + b] W_16^(n1 k1).
-// xy_nat2perm: natural order in; step 1 (item (b, l)) a DFT-16 over the rows
-// b + B n1 and the twiddles W_P^(b k1), written back to the same rows; step 2 (item
-// (k1, l)) a DFT-B over rows k1 B .. k1 B + B - 1.  X[k] is left in row
-// xy_row<B>(k) = (k mod 16) B + k / 16.
-// xy_perm2nat: the same split run backwards, from that order to natural order:
-// step 1 (item (k1, l)) a DFT-B over rows k1 B + k2, then W_P^(b k1); step 2 (item
-// (b, l)) a DFT-16 over rows k1 B + b, x[b + B n1] written to row b + B n1.
-// One barrier after each step, as in the Stockham stages (one per radix stage).
-template <int B>
-__device__ __forceinline__ int xy_row(int k) { return (k & 15) * B + (k >> 4); }
+// Step 1 runs straight from HBM: item (b, l) loads x[b + B n1] (n1 < 16) of column
+// l, does the DFT-16, multiplies by W_P^(b k1) and writes LDS row b + B k1
+// (element (row r, line l) at s[r * TK + l]).  After one barrier, step 2 in
+// registers: item (k1, l) reads rows k1 B .. k1 B + B-1 and does the DFT-B, leaving
+// X[k1 + 16 k2] in v[k2].
+// FY stores X straight to HBM (row k1 + 16 k2).  IY stores output row o, which reads
+// index j = (o + off) mod P, from the item holding j.  X multiplies by the kernel
+// spectrum in the same registers, runs the inverse DFT-B and twiddles W_P^(-b k1),
+// writes the rows back, and after one barrier the inverse DFT-16 of item (b, l)
+// leaves x[b + B n1] in registers, stored the same way.
+// Reference-mode fold: rows o < foldn also take index (o + N + off) mod P, which
+// another item holds.  Both operands go to a small LDS buffer past the tile, primary
+// rows at fb[o * TK + l] and folded rows at fb[(foldn + o) * TK + l], and after one
+// barrier row o gets primary + folded (the order of the LDS form of the pass).  The
+// host reserves 2 * foldn * TK elements for it.
 // block bound of the two-step instances (their item loops are strided, so the host
 // caps the block at this; the register budget per instance is xy_waves())
 constexpr int kXyThreads = PVD_XY_MAX_BLOCK;
-
-template <int B, bool INV>
-__device__ __forceinline__ void xy_nat2perm(float2 *s, int lnl, const float2 *tw) {
-    const int TK = 1 << lnl;
-    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
-        const int b = w >> lnl, l = w & (TK - 1);
-        float2 v[16];
-#pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = s[(b + B * n1) * TK + l];
-        Dft<16, INV>::run(v);
-#pragma unroll
-        for (int k1 = 1; k1 < 16; ++k1) {
-            float2 t = tw[b * k1];
-            if (INV) t.y = -t.y;
-            v[k1] = cmul(v[k1], t);
-        }
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) s[(b + B * k1) * TK + l] = v[k1];
-    }
-    lds_barrier();
-    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
-        const int k1 = w >> lnl, l = w & (TK - 1);
-        float2 v[B];
-#pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) v[n2] = s[(k1 * B + n2) * TK + l];
-        dft_any<B, INV>(v);
-#pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = v[k2];
-    }
-    lds_barrier();
-}
-
-template <int B, bool INV>
-__device__ __forceinline__ void xy_perm2nat(float2 *s, int lnl, const float2 *tw) {
-    const int TK = 1 << lnl;
-    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
-        const int k1 = w >> lnl, l = w & (TK - 1);
-        float2 v[B];
-#pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) v[k2] = s[(k1 * B + k2) * TK + l];
-        dft_any<B, INV>(v);
-#pragma unroll
-        for (int b = 1; b < B; ++b) {
-            float2 t = tw[b * k1];
-            if (INV) t.y = -t.y;
-            v[b] = cmul(v[b], t);
-        }
-#pragma unroll
-        for (int b = 0; b < B; ++b) s[(k1 * B + b) * TK + l] = v[b];
-    }
-    lds_barrier();
-    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
-        const int b = w >> lnl, l = w & (TK - 1);
-        float2 v[16];
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) v[k1] = s[(k1 * B + b) * TK + l];
-        Dft<16, INV>::run(v);
-#pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = v[n1];
-    }
-    lds_barrier();
-}
-
-// store_col_tile for a tile left in xy_row<B> order: FFT index k sits in LDS row
-// xy_row<B>(k) (same output mapping: row o takes index (o + off) mod P, plus index
-// (o + rows_out + off) mod P when o < foldn).
-template <int B>
-__device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long long pitch,
-                                                    const float2 *__restrict__ s, int rows_out, int lnl, int P,
-                                                    int off, int foldn, int nc) {
-    const int lp = lnl - 1;
-    const int total = rows_out << lp;
-    const int pmask = (1 << lp) - 1;
-    const float4 *s4 = reinterpret_cast<const float4 *>(s);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int o = e >> lp, cp = e & pmask;
-        if ((cp << 1) >= nc) continue;
-        float4 val = s4[(xy_row<B>(wrap_add(o, off, P)) << lp) + cp];
-        if (o < foldn) {
-            const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
-            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
-        }
-        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
-    }
-}
-
-// ---------------------------------------------------------------------------------
-// Fused register two-step passes (P = 16 B), one LDS exchange per transform:
-//   step 1 straight from HBM: item (b, l) loads x[b + B n1] (n1 < 16) of column l,
-//     DFT-16, twiddles W_P^(b k1), writes LDS row b + B k1 (= k1 B + b ordering of
-//     xy_nat2perm after its first step);
-//   step 2 in registers: item (k1, l) reads rows k1 B .. k1 B + B-1 and does the
-//     DFT-B: X[k1 + 16 k2] in v[k2].
-// FY stores X straight to HBM (row k1 + 16 k2).  X multiplies by the kernel spectrum
-// in the same registers, runs the inverse DFT-B and twiddles W_P^(-b k1) (the first
-// step of xy_perm2nat), writes the rows back, and after one barrier the inverse DFT-16
-// of item (b, l) leaves x[b + B n1] in registers.  Outputs with no reference-mode fold
-// go straight to HBM (row o reads index (o + off) mod P); with a fold the tile is
-// written back to LDS and stored by the row maps of store_col_tile(_perm).
 
 // step 1 from HBM (all loads of an item issued before any is used; rows >= rows_in and
 // columns >= nc are zero); entered after a barrier that published tw
@@ -788,8 +694,32 @@ __device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl,
     }
 }
 
-// output row o of FFT index j: (j - off) mod P
-__device__ __forceinline__ int out_row(int j, int off, int P) { return j >= off ? j - off : j + P - off; }
+// Output of FFT index j (column l of a tile, row pitch `pitch`): row (j - off) mod P
+// when below rows_out, except the rows o < foldn and their folded partners, which go
+// to the fold buffer fb.
+__device__ __forceinline__ void xy_put(float2 *__restrict__ g, long long pitch, float2 *__restrict__ fb, int lnl,
+                                       int j, int l, int nc, const pvd_axis_d &ax, float2 val) {
+    const int o = j >= ax.off ? j - ax.off : j + ax.n - ax.off;
+    if (o < ax.N) {
+        if (o < ax.foldn) fb[(o << lnl) + l] = val;
+        else if (l < nc) g[(long long)o * pitch + l] = val;
+    } else if (o - ax.N < ax.foldn) {
+        fb[((ax.foldn + o - ax.N) << lnl) + l] = val;
+    }
+}
+
+// rows o < foldn: primary + folded, after the barrier that published fb
+__device__ __forceinline__ void xy_fold_store(float2 *__restrict__ g, long long pitch,
+                                              const float2 *__restrict__ fb, int lnl, int nc, int foldn) {
+    const int TK = 1 << lnl;
+    for (int e = threadIdx.x; e < (foldn << lnl); e += blockDim.x) {
+        const int o = e >> lnl, l = e & (TK - 1);
+        if (l < nc) {
+            const float2 a = fb[e], f = fb[e + (foldn << lnl)];
+            g[(long long)o * pitch + l] = make_float2(a.x + f.x, a.y + f.y);
+        }
+    }
+}
 
 template <int B, bool INV>
 __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
@@ -801,34 +731,30 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int c0 = (tile - x * ntc) << lnl;
     const int nc = min(TK, H - c0);
     float2 *g = v.work + (long long)x * v.wplane + c0;
+    float2 *fb = s + Py * TK;
     load_twiddles(tw, ay);
     lds_barrier();
     xy_step1_global<B, INV>(s, lnl, tw, g, Hp, INV ? Py : Ny, nc);
     lds_barrier();
-    const bool direct = !INV || ay.foldn == 0;     // block-uniform
     for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
         const int k1 = w >> lnl, l = w & (TK - 1);
         float2 u[B];
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) u[n2] = s[(k1 * B + n2) * TK + l];
         dft_any<B, INV>(u);
-        if (direct) {
+        if (!INV) {
             if (l < nc) {
 #pragma unroll
-                for (int k2 = 0; k2 < B; ++k2) {
-                    const int j = k1 + 16 * k2;
-                    const int o = INV ? out_row(j, ay.off, Py) : j;
-                    if (o < (INV ? Ny : Py)) g[(long long)o * Hp + l] = u[k2];
-                }
+                for (int k2 = 0; k2 < B; ++k2) g[(long long)(k1 + 16 * k2) * Hp + l] = u[k2];
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = u[k2];
+            for (int k2 = 0; k2 < B; ++k2) xy_put(g, Hp, fb, lnl, k1 + 16 * k2, l, nc, ay, u[k2]);
         }
     }
-    if (!direct) {
+    if (INV && ay.foldn) {
         lds_barrier();
-        store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
+        xy_fold_store(g, Hp, fb, lnl, nc, ay.foldn);
     }
 }
 
@@ -847,6 +773,7 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const long long splane = (long long)Py * Hp;      // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
+    float2 *fb = s + Px * TK;
     const float2 *sp = v.spec + off0;
     const float *sr = reinterpret_cast<const float *>(v.spec);
     const long long pyh = Py / 2 + 1;
@@ -889,31 +816,19 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         for (int b = 0; b < B; ++b) s[(k1 * B + b) * TK + l] = u[b];
     }
     lds_barrier();
-    // inverse DFT-16 of item (b, l): x[b + B n1] in registers
-    const bool direct = ax.foldn == 0;               // block-uniform
+    // inverse DFT-16 of item (b, l): x[b + B n1] in registers, stored row by row
     for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
         const int b = w >> lnl, l = w & (TK - 1);
         float2 u[16];
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) u[k1] = s[(k1 * B + b) * TK + l];
         Dft<16, true>::run(u);
-        if (direct) {
-            if (l < nc) {
 #pragma unroll
-                for (int n1 = 0; n1 < 16; ++n1) {
-                    const int o = out_row(b + B * n1, ax.off, Px);
-                    if (o < Nx) g[(long long)o * plane + l] = u[n1];
-                }
-            }
-        } else {
-            // rows b + B n1 are read only by this item: write back in natural order
-#pragma unroll
-            for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = u[n1];
-        }
+        for (int n1 = 0; n1 < 16; ++n1) xy_put(g, plane, fb, lnl, b + B * n1, l, nc, ax, u[n1]);
     }
-    if (!direct) {
+    if (ax.foldn) {
         lds_barrier();
-        store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
+        xy_fold_store(g, plane, fb, lnl, nc, ax.foldn);
     }
 }
 
@@ -1091,11 +1006,12 @@ __device__ __forceinline__ void run_batch(const pvd_batch_args &a) {
 // The y/x register two-step for P = 16 * B has its own kernels, so that their
 // register budget does not touch the other instances.  Waves per SIMD (exactly; the
 // VGPR cap is 512 / waves): FY/IY hold 16 loads or a B-point line per item and keep
-// four up to B = 27; X also holds the B spectrum values of its line through the
-// forward transform and keeps four up to B = 21, three up to 30 and two above (a
-// 512-thread block of the 512-point X tile is one block per CU either way).
+// four up to B = 27, three up to 32 and two above; X also holds the B spectrum values
+// of its line through the forward transform and keeps four up to B = 21, three up to
+// 30 and two above.  A 512-thread block of a 512- or 560-point 16-column tile (over
+// 64 KB of LDS) is one block per CU, two waves per SIMD, either way.
 template <int PASS, int B> constexpr int xy_waves() {
-    return PASS == PVD_PASS_X ? (B <= 21 ? 4 : B <= 30 ? 3 : 2) : (B <= 27 ? 4 : 3);
+    return PASS == PVD_PASS_X ? (B <= 21 ? 4 : B <= 30 ? 3 : 2) : (B <= 27 ? 4 : B <= 32 ? 3 : 2);
 }
 }  // namespace
 

@@ -321,7 +321,7 @@ constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25, 27, 1
 #define PVD_XY_MAX_BLOCK 512
 // y/x lengths 16 * B other than 256 with a register two-step instance, grouped by
 // the source file that compiles them (pvd_fft_xy1/2/3.hip)
-#define PVD_XY_BS_1(X) X(6) X(8) X(9) X(10) X(12) X(14)
+#define PVD_XY_BS_1(X) X(6) X(8) X(9) X(10) X(12) X(14) X(16)
 #define PVD_XY_BS_2(X) X(15) X(18) X(20) X(21) X(24) X(25)
 #define PVD_XY_BS_3(X) X(27) X(28) X(30) X(32) X(35) X(36)
 #define PVD_XY_BS(X) PVD_XY_BS_1(X) PVD_XY_BS_2(X) PVD_XY_BS_3(X)

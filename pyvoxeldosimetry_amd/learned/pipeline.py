@@ -28,7 +28,7 @@ from .mc import MonteCarlo
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 PRETRAINED = {("Y90", 2.0): "cnn_y90_2mm.safetensors"}
-EVAL_SEEDS_TEST = range(20000, 20006)      # held-out phantoms (training used 10000..10299)
+EVAL_SEEDS_TEST = range(20000, 20006)      # held-out phantoms (training used seeds 10000..11999)
 
 
 def synth_phantom(seed: int, shape=(128, 128, 128), device=None):

@@ -108,7 +108,7 @@ def _smooth(n):
 
 
 # register transforms: 256 = 16 x 16 on y/x, P = 16 B on y/x, P = A B on z
-XY_TWO_STEP = {16 * b for b in (6, 8, 9, 10, 12, 14, 15, 18, 20, 21, 24, 25, 27, 28, 30, 32, 35, 36)}
+XY_TWO_STEP = {16 * b for b in (6, 8, 9, 10, 12, 14, 15, 16, 18, 20, 21, 24, 25, 27, 28, 30, 32, 35, 36)}
 Z_TWO_STEP = {8 * b for b in (16, 20, 24, 25, 30, 12, 14, 15, 18, 21, 27)} | {16 * b for b in (16, 20, 25, 27, 14, 18, 21,
                                                                                              24, 30)}
 
