@@ -125,8 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 error check")
-    ap.add_argument("--no-eval", action="store_true", help="cnn config: skip the Monte Carlo error evaluation")
-    ap.add_argument("--no-cnn", action="store_true", help="ct config: leave the CNN-corrected part out of the line")
+    ap.add_argument("--no-eval", action="store_true", help="cnn and torso configs: skip the Monte Carlo error evaluation")
+    ap.add_argument("--no-cnn", action="store_true",
+                    help="ct and torso configs: leave the CNN-corrected part out of the line")
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 256 MiB (reference for rocprof byte counters)")

@@ -319,13 +319,14 @@ int ilog2(int v) {
     return r;
 }
 
-// The register-resident 256-point X pass addresses the workspace and the kernel
-// spectrum with 32-bit byte offsets from a tile's base: the x-plane walk reaches
-// (Nx - 1) * wplane * 8 bytes, the general spectrum 255 * Py * Hp * 8 bytes (the
-// real octant less).  Larger volumes take the generic X pass (64-bit offsets).
-bool x_sq256_offsets_fit(long long Nx, long long Py, long long Hp, long long wplane) {
-    const long long lim = 0xffffffffLL - 256;
-    return (Nx - 1) * wplane * 8 <= lim && 255LL * Py * Hp * 8 <= lim;
+// The register-resident X passes (256-point and fused two-step) address the workspace
+// and the kernel spectrum with 32-bit byte offsets from a tile's base: the x-plane walk
+// reaches (Nx - 1) * wplane * 8 bytes, the general spectrum (Px - 1) * Py * Hp * 8
+// bytes (the real octant less), plus a tile's columns.  Larger volumes take the
+// generic X pass (64-bit offsets).
+bool x_offsets_fit(long long Nx, long long Px, long long Py, long long Hp, long long wplane) {
+    const long long lim = 0xffffffffLL - 256 - 8 * Hp;
+    return (Nx - 1) * wplane * 8 <= lim && (Px - 1) * Py * Hp * 8 <= lim;
 }
 
 // Host-only layout of an FFT plan: tiles, register paths, launch shapes and buffer
@@ -348,8 +349,9 @@ int fft_layout(pvd_plan *p) {
     const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
     // y/x register paths: 1 = the 256-point path (16-line tiles, no reference-mode fold;
     // checked again in the kernel); 2 = the two-step P = 16 * rb (rb in kXyRegB) in its
-    // own kernel instances, which keep 2 * foldn * TK elements of LDS past the tile for
-    // the fold (fewer lines per tile when that would pass the LDS budget)
+    // own kernel instances; the fused form (rb < PVD_XY_LDS_B) keeps 2 * foldn * TK
+    // elements of LDS past the tile for the fold (fewer lines per tile when that would
+    // pass the LDS budget)
     for (int a = 0; a < 2; ++a) {
         pvd_axis_d &ax = v.ax[a];
         int &tk = a ? tky : tkx;
@@ -363,7 +365,8 @@ int fft_layout(pvd_plan *p) {
             ax.ra = 16;
             ax.ra_log = 4;
             ax.rb = ax.n / 16;
-            auto fold = [&](int t) { return sizeof(float2) * 2 * (size_t)ax.foldn * (size_t)t; };
+            const bool fused = ax.rb < PVD_XY_LDS_B;
+            auto fold = [&](int t) { return fused ? sizeof(float2) * 2 * (size_t)ax.foldn * (size_t)t : 0; };
             while (tk > 2 && ld + fold(tk) > kFftLdsCap) pick_tile(ax.n, tk / 2, 0, &tk, &bl, &ld);
             ld += fold(tk);
         }
@@ -399,14 +402,12 @@ int fft_layout(pvd_plan *p) {
     // x-planes of the workspace are Py * Hp complex plus PVD_PLANE_PAD (0: padding by one
     // or three 128-byte lines measured the same on the CT workload)
     v.wplane = Py * v.Hp + PVD_PLANE_PAD;
-    // the 256-point X pass addresses with 32-bit offsets: larger volumes take the
-    // two-step instance for 256 (64-bit offsets; no fold, so no extra LDS)
-    if (Px == 256 && !x_sq256_offsets_fit(Nx, Py, v.Hp, v.wplane) && v.ax[0].reg == 1) {
-        v.ax[0].reg = 2;
-        v.ax[0].ra = 16;
-        v.ax[0].ra_log = 4;
-        v.ax[0].rb = 16;
-        bx = std::min(bx, PVD_XY_MAX_BLOCK);
+    // 32-bit offsets of the register X passes (the LDS-resident two-step and the generic
+    // pass address with 64 bits)
+    {
+        pvd_axis_d &ax = v.ax[0];
+        const bool reg32 = (ax.reg == 1 && Px == 256) || (ax.reg == 2 && ax.rb < PVD_XY_LDS_B);
+        if (reg32 && !x_offsets_fit(Nx, Px, Py, v.Hp, v.wplane)) ax.reg = 0, ax.ra = ax.ra_log = ax.rb = 0;
     }
     const long long nlines = (long long)Nx * Ny;
     const long long tz = (nlines + 2 * lz - 1) / (2 * lz);

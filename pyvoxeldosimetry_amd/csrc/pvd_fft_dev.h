@@ -668,17 +668,24 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 // caps the block at this; the register budget per instance is xy_waves())
 constexpr int kXyThreads = PVD_XY_MAX_BLOCK;
 
+// Addressing: every global access is a 32-bit byte offset from the tile's wave-uniform
+// base (one VGPR, no 64-bit address math: at()).  The host takes the fused form only
+// where the X pass's workspace walk, (Nx - 1) * wplane * 8 bytes, and the general
+// spectrum, (Px - 1) * Py * Hp * 8 bytes, stay below 4 GiB (xy_offsets_fit); the
+// y passes stay inside one x-plane.
+
 // step 1 from HBM (all loads of an item issued before any is used; rows >= rows_in and
 // columns >= nc are zero); entered after a barrier that published tw
 template <int B, bool INV>
 __device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl, const float2 *__restrict__ tw,
-                                                const float2 *__restrict__ g, long long pitch, int rows_in, int nc) {
+                                                const float2 *__restrict__ g, uint32_t pitch_b, int rows_in, int nc) {
     const int TK = 1 << lnl;
     for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
         const int b = w >> lnl, l = w & (TK - 1);
+        const uint32_t lb = (uint32_t)l * 8u;
         float2 v[16];
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = g[(long long)min(b + B * n1, rows_in - 1) * pitch + l];
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = at(g, (uint32_t)min(b + B * n1, rows_in - 1) * pitch_b + lb);
 #pragma unroll
         for (int n1 = 0; n1 < 16; ++n1)
             if (b + B * n1 >= rows_in || l >= nc) v[n1] = make_float2(0.f, 0.f);
@@ -694,29 +701,29 @@ __device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl,
     }
 }
 
-// Output of FFT index j (column l of a tile, row pitch `pitch`): row (j - off) mod P
-// when below rows_out, except the rows o < foldn and their folded partners, which go
-// to the fold buffer fb.
-__device__ __forceinline__ void xy_put(float2 *__restrict__ g, long long pitch, float2 *__restrict__ fb, int lnl,
+// Output of FFT index j (column l of a tile, rows pitch_b bytes apart): row
+// (j - off) mod P when below rows_out, except the rows o < foldn and their folded
+// partners, which go to the fold buffer fb.
+__device__ __forceinline__ void xy_put(float2 *__restrict__ g, uint32_t pitch_b, float2 *__restrict__ fb, int lnl,
                                        int j, int l, int nc, const pvd_axis_d &ax, float2 val) {
     const int o = j >= ax.off ? j - ax.off : j + ax.n - ax.off;
     if (o < ax.N) {
         if (o < ax.foldn) fb[(o << lnl) + l] = val;
-        else if (l < nc) g[(long long)o * pitch + l] = val;
+        else if (l < nc) at(g, (uint32_t)o * pitch_b + (uint32_t)l * 8u) = val;
     } else if (o - ax.N < ax.foldn) {
         fb[((ax.foldn + o - ax.N) << lnl) + l] = val;
     }
 }
 
 // rows o < foldn: primary + folded, after the barrier that published fb
-__device__ __forceinline__ void xy_fold_store(float2 *__restrict__ g, long long pitch,
+__device__ __forceinline__ void xy_fold_store(float2 *__restrict__ g, uint32_t pitch_b,
                                               const float2 *__restrict__ fb, int lnl, int nc, int foldn) {
     const int TK = 1 << lnl;
     for (int e = threadIdx.x; e < (foldn << lnl); e += blockDim.x) {
         const int o = e >> lnl, l = e & (TK - 1);
         if (l < nc) {
             const float2 a = fb[e], f = fb[e + (foldn << lnl)];
-            g[(long long)o * pitch + l] = make_float2(a.x + f.x, a.y + f.y);
+            at(g, (uint32_t)o * pitch_b + (uint32_t)l * 8u) = make_float2(a.x + f.x, a.y + f.y);
         }
     }
 }
@@ -730,11 +737,12 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int x = tile / ntc;
     const int c0 = (tile - x * ntc) << lnl;
     const int nc = min(TK, H - c0);
-    float2 *g = v.work + (long long)x * v.wplane + c0;
+    float2 *g = v.work + (long long)x * v.wplane + c0;    // uniform base of the tile
+    const uint32_t pitch_b = (uint32_t)Hp * 8u;
     float2 *fb = s + Py * TK;
     load_twiddles(tw, ay);
     lds_barrier();
-    xy_step1_global<B, INV>(s, lnl, tw, g, Hp, INV ? Py : Ny, nc);
+    xy_step1_global<B, INV>(s, lnl, tw, g, pitch_b, INV ? Py : Ny, nc);
     lds_barrier();
     for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
         const int k1 = w >> lnl, l = w & (TK - 1);
@@ -744,17 +752,18 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         dft_any<B, INV>(u);
         if (!INV) {
             if (l < nc) {
+                const uint32_t o0 = (uint32_t)k1 * pitch_b + (uint32_t)l * 8u;
 #pragma unroll
-                for (int k2 = 0; k2 < B; ++k2) g[(long long)(k1 + 16 * k2) * Hp + l] = u[k2];
+                for (int k2 = 0; k2 < B; ++k2) at(g, o0 + (uint32_t)k2 * 16u * pitch_b) = u[k2];
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) xy_put(g, Hp, fb, lnl, k1 + 16 * k2, l, nc, ay, u[k2]);
+            for (int k2 = 0; k2 < B; ++k2) xy_put(g, pitch_b, fb, lnl, k1 + 16 * k2, l, nc, ay, u[k2]);
         }
     }
     if (INV && ay.foldn) {
         lds_barrier();
-        xy_fold_store(g, Hp, fb, lnl, nc, ay.foldn);
+        xy_fold_store(g, pitch_b, fb, lnl, nc, ay.foldn);
     }
 }
 
@@ -769,18 +778,17 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int y = x_tile_row(u0, Py);
     const int c0 = (tile - u0 * ntc) << lnl;
     const int nc = min(TK, H - c0);
-    const long long plane = v.wplane;                 // work
-    const long long splane = (long long)Py * Hp;      // spectrum
     const long long off0 = (long long)y * Hp + c0;
-    float2 *g = v.work + off0;
-    float2 *fb = s + Px * TK;
+    float2 *g = v.work + off0;                                   // uniform bases of the tile
     const float2 *sp = v.spec + off0;
-    const float *sr = reinterpret_cast<const float *>(v.spec);
-    const long long pyh = Py / 2 + 1;
-    const long long row0 = (long long)min(y, Py - y) * Hp + c0;
+    const float *sr = reinterpret_cast<const float *>(v.spec) + (long long)min(y, Py - y) * Hp + c0;
+    const uint32_t pitch_b = (uint32_t)v.wplane * 8u;             // x-plane step of the workspace
+    const uint32_t kplane_b = (uint32_t)Py * (uint32_t)Hp * 8u;   // kx step of the general spectrum
+    const uint32_t krow_b = (uint32_t)(Py / 2 + 1) * (uint32_t)Hp * 4u;   // kx step of the real octant
+    float2 *fb = s + Px * TK;
     load_twiddles(tw, ax);
     lds_barrier();
-    xy_step1_global<B, false>(s, lnl, tw, g, plane, Nx, nc);
+    xy_step1_global<B, false>(s, lnl, tw, g, pitch_b, Nx, nc);
     lds_barrier();
     // forward DFT-B, times the spectrum, inverse DFT-B and twiddles, in registers
     for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
@@ -793,11 +801,12 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 #pragma unroll
             for (int k2 = 0; k2 < B; ++k2) {
                 const int k = k1 + 16 * k2;
-                sv[k2] = make_float2(sr[ok ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0], 0.f);
+                sv[k2] = make_float2(at(sr, ok ? (uint32_t)min(k, Px - k) * krow_b + (uint32_t)l * 4u : 0u), 0.f);
             }
         } else {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) sv[k2] = sp[ok ? (long long)(k1 + 16 * k2) * splane + l : 0];
+            for (int k2 = 0; k2 < B; ++k2)
+                sv[k2] = at(sp, ok ? (uint32_t)(k1 + 16 * k2) * kplane_b + (uint32_t)l * 8u : 0u);
         }
         float2 u[B];
 #pragma unroll
@@ -824,12 +833,195 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
         for (int k1 = 0; k1 < 16; ++k1) u[k1] = s[(k1 * B + b) * TK + l];
         Dft<16, true>::run(u);
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) xy_put(g, plane, fb, lnl, b + B * n1, l, nc, ax, u[n1]);
+        for (int n1 = 0; n1 < 16; ++n1) xy_put(g, pitch_b, fb, lnl, b + B * n1, l, nc, ax, u[n1]);
     }
     if (ax.foldn) {
         lds_barrier();
-        xy_fold_store(g, plane, fb, lnl, nc, ax.foldn);
+        xy_fold_store(g, pitch_b, fb, lnl, nc, ax.foldn);
     }
+}
+
+// LDS-resident form of the two-step, for B >= kXyLdsB (P = 560, 576): a 16-column tile
+// of 35 or 36 B-rows has more step-1 items than the 512-thread block, so the fused
+// form above would load in two rounds; here the tile is loaded into LDS by all
+// threads, both steps run over LDS with one barrier each, and the result leaves
+// through a row map.  xy_nat2perm: natural order in; step 1 (item (b, l)) a DFT-16
+// over the rows b + B n1 and the twiddles W_P^(b k1), written back to the same rows;
+// step 2 (item (k1, l)) a DFT-B over rows k1 B .. k1 B + B - 1.  X[k] is left in row
+// xy_row<B>(k) = (k mod 16) B + k / 16.  xy_perm2nat: the same split run backwards,
+// from that order to natural order.
+constexpr int kXyLdsB = PVD_XY_LDS_B;
+template <int B>
+__device__ __forceinline__ int xy_row(int k) { return (k & 15) * B + (k >> 4); }
+template <int B, bool INV>
+__device__ __forceinline__ void xy_nat2perm(float2 *s, int lnl, const float2 *tw) {
+    const int TK = 1 << lnl;
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = s[(b + B * n1) * TK + l];
+        Dft<16, INV>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+            float2 t = tw[b * k1];
+            if (INV) t.y = -t.y;
+            v[k1] = cmul(v[k1], t);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) s[(b + B * k1) * TK + l] = v[k1];
+    }
+    lds_barrier();
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        float2 v[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = s[(k1 * B + n2) * TK + l];
+        dft_any<B, INV>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = v[k2];
+    }
+    lds_barrier();
+}
+
+template <int B, bool INV>
+__device__ __forceinline__ void xy_perm2nat(float2 *s, int lnl, const float2 *tw) {
+    const int TK = 1 << lnl;
+    for (int w = threadIdx.x; w < (16 << lnl); w += blockDim.x) {
+        const int k1 = w >> lnl, l = w & (TK - 1);
+        float2 v[B];
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k2] = s[(k1 * B + k2) * TK + l];
+        dft_any<B, INV>(v);
+#pragma unroll
+        for (int b = 1; b < B; ++b) {
+            float2 t = tw[b * k1];
+            if (INV) t.y = -t.y;
+            v[b] = cmul(v[b], t);
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) s[(k1 * B + b) * TK + l] = v[b];
+    }
+    lds_barrier();
+    for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
+        const int b = w >> lnl, l = w & (TK - 1);
+        float2 v[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = s[(k1 * B + b) * TK + l];
+        Dft<16, INV>::run(v);
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = v[n1];
+    }
+    lds_barrier();
+}
+
+// store_col_tile for a tile left in xy_row<B> order: FFT index k sits in LDS row
+// xy_row<B>(k) (same output mapping: row o takes index (o + off) mod P, plus index
+// (o + rows_out + off) mod P when o < foldn).
+template <int B>
+__device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long long pitch,
+                                                    const float2 *__restrict__ s, int rows_out, int lnl, int P,
+                                                    int off, int foldn, int nc) {
+    const int lp = lnl - 1;
+    const int total = rows_out << lp;
+    const int pmask = (1 << lp) - 1;
+    const float4 *s4 = reinterpret_cast<const float4 *>(s);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int o = e >> lp, cp = e & pmask;
+        if ((cp << 1) >= nc) continue;
+        float4 val = s4[(xy_row<B>(wrap_add(o, off, P)) << lp) + cp];
+        if (o < foldn) {
+            const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
+            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
+        }
+        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
+    }
+}
+
+template <int B, bool INV>
+__device__ void pass_y_2s_lds(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ay = v.ax[1];
+    const int Ny = ay.N, Py = ay.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltky, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
+    const int x = tile / ntc;
+    const int c0 = (tile - x * ntc) << lnl;
+    const int nc = min(TK, H - c0);
+    float2 *g = v.work + (long long)x * v.wplane + c0;
+    load_twiddles(tw, ay);
+    load_col_tile(s, g, Hp, Py, INV ? Py : Ny, lnl, nc);
+    lds_barrier();
+    xy_nat2perm<B, INV>(s, lnl, tw);
+    if (!INV) store_col_tile_perm<B>(g, Hp, s, Py, lnl, Py, 0, 0, nc);
+    else store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
+}
+
+template <int B>
+__device__ void pass_x_2s_lds(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
+    const pvd_axis_d &ax = v.ax[0];
+    const int Nx = ax.N, Px = ax.n, H = v.H, Hp = v.Hp;
+    const int lnl = v.ltkx, TK = 1 << lnl;
+    const int ntc = (H + TK - 1) >> lnl;
+    const int u0 = tile / ntc;
+    const int y = x_tile_row(u0, v.ax[1].n);
+    const int c0 = (tile - u0 * ntc) << lnl;
+    const int nc = min(TK, H - c0);
+    const long long plane = v.wplane;                 // work
+    const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
+    const long long off0 = (long long)y * Hp + c0;
+    float2 *g = v.work + off0;
+    const float2 *sp = v.spec + off0;
+    // After the forward two-step, LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2.
+    // Every spectrum load is issued at a clamped address (the tile's first entry where
+    // masked) and masked afterwards.  (Loading the first batch before the forward
+    // transform took the 35-point instance from 131 to 170 VGPRs and X from 471 to
+    // 735 us on the 512x512x400 torso: it halved the workgroups per CU.)
+    const int total = Px << lnl;
+    const float *sr = reinterpret_cast<const float *>(v.spec);
+    const long long pyh = v.ax[1].n / 2 + 1;
+    const long long row0 = (long long)min(y, v.ax[1].n - y) * Hp + c0;
+    auto sym_index = [&](int e) -> long long {
+        const int r = e >> lnl, l = e & (TK - 1);
+        const int k = r / B + 16 * (r % B);
+        return (e < total && l < nc) ? (long long)min(k, Px - k) * pyh * Hp + row0 + l : row0;
+    };
+    load_twiddles(tw, ax);
+    load_col_tile(s, g, plane, Px, Nx, lnl, nc);
+    lds_barrier();
+    xy_nat2perm<B, false>(s, lnl, tw);
+    if (v.sym) {
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float sv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) sv[u] = sr[sym_index(e0 + u * blockDim.x)];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cscale(s[e], ok ? sv[u] : 0.f);
+            }
+        }
+    } else {
+        for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * blockDim.x) {
+            float2 sv[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const int r = e >> lnl, l = e & (TK - 1);
+                const int k = r / B + 16 * (r % B);
+                sv[u] = sp[(e < total && l < nc) ? k * splane + l : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = e0 + u * blockDim.x;
+                const bool ok = e < total && (e & (TK - 1)) < nc;
+                if (e < total) s[e] = cmul(s[e], ok ? sv[u] : make_float2(0.f, 0.f));
+            }
+        }
+    }
+    lds_barrier();
+    xy_perm2nat<B, true>(s, lnl, tw);
+    store_col_tile(g, plane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
 }
 
 // ---------------------------------------------------------------------------------
@@ -950,7 +1142,12 @@ __device__ __forceinline__ void run_pass(const pvd_vol_d &v, int tile) {
     const int axis = (PASS == PVD_PASS_X) ? 0 : (PASS == PVD_PASS_FY || PASS == PVD_PASS_IY) ? 1 : 2;
     float2 *tw = smem;
     float2 *s = smem + ((v.ax[axis].n + 7) & ~7);
-    if constexpr (XB > 0) {           // register two-step, P = 16 * XB (host-selected instance)
+    if constexpr (XB >= kXyLdsB) {    // register two-step over an LDS tile, P = 16 * XB
+        if (PASS == PVD_PASS_FY) pass_y_2s_lds<XB, false>(v, tile, tw, s);
+        else if (PASS == PVD_PASS_IY) pass_y_2s_lds<XB, true>(v, tile, tw, s);
+        else if (PASS == PVD_PASS_X) pass_x_2s_lds<XB>(v, tile, tw, s);
+        return;
+    } else if constexpr (XB > 0) {    // fused register two-step, P = 16 * XB
         if (PASS == PVD_PASS_FY) pass_y_2s<XB, false>(v, tile, tw, s);
         else if (PASS == PVD_PASS_IY) pass_y_2s<XB, true>(v, tile, tw, s);
         else if (PASS == PVD_PASS_X) pass_x_2s<XB>(v, tile, tw, s);
@@ -1010,8 +1207,15 @@ __device__ __forceinline__ void run_batch(const pvd_batch_args &a) {
 // of its line through the forward transform and keeps four up to B = 21, three up to
 // 30 and two above.  A 512-thread block of a 512- or 560-point 16-column tile (over
 // 64 KB of LDS) is one block per CU, two waves per SIMD, either way.
-template <int PASS, int B> constexpr int xy_waves() {
+// The LDS-resident instances (B >= 35) hold only one B-point line per item and ask for
+// at least four, the batch X at least three (its descriptor lookup would spill at 128
+// VGPRs); with an exact count there the compiler spilled 2-10 VGPRs.
+template <int PASS, int B, bool BATCH> constexpr int xy_waves() {
+    if (B >= kXyLdsB) return (BATCH && PASS == PVD_PASS_X) ? 3 : 4;
     return PASS == PVD_PASS_X ? (B <= 21 ? 4 : B <= 30 ? 3 : 2) : (B <= 27 ? 4 : B <= 32 ? 3 : 2);
+}
+template <int PASS, int B, bool BATCH> constexpr int xy_waves_max() {
+    return B >= kXyLdsB ? 8 : xy_waves<PASS, B, BATCH>();
 }
 }  // namespace
 
@@ -1036,7 +1240,8 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass_batch(pvd_batch
 }
 
 template <int PASS, int B>
-__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B>(), xy_waves<PASS, B>())))
+__global__ __launch_bounds__(kXyThreads)
+__attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B, false>(), xy_waves_max<PASS, B, false>())))
 void pvd_fft_pass_xy(pvd_vol_d v) {
     int tile = blockIdx.x;
     if (PASS == PVD_PASS_X) {
@@ -1047,7 +1252,8 @@ void pvd_fft_pass_xy(pvd_vol_d v) {
 }
 
 template <int PASS, int B>
-__global__ __launch_bounds__(kXyThreads) __attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B>(), xy_waves<PASS, B>())))
+__global__ __launch_bounds__(kXyThreads)
+__attribute__((amdgpu_waves_per_eu(xy_waves<PASS, B, true>(), xy_waves_max<PASS, B, true>())))
 void pvd_fft_pass_xy_batch(pvd_batch_args a) {
     run_batch<PASS, false, 16, B>(a);
 }

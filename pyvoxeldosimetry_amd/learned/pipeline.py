@@ -113,8 +113,10 @@ class LearnedCorrector:
         from safetensors import safe_open
         with safe_open(path, framework="pt") as f:
             file_meta = dict(f.metadata() or {})
-        # the Monte Carlo the weights were trained against (files without the key: electrons only)
-        meta = dict(bound=0.9, photons=file_meta.get("mc_photons", "0") == "1")
+        # the Monte Carlo the weights were trained against (files without the key: electrons
+        # only) and the grid of the point kernel of their plain dose (without the key: 13)
+        meta = dict(bound=0.9, photons=file_meta.get("mc_photons", "0") == "1",
+                    kernel_size=int(file_meta.get("kernel_size", "13")))
         meta.update(kw)
         c = cls(nuclide, voxel_mm=voxel_mm, **meta)
         c.weights_meta = c.net.load_weights(path)
@@ -145,13 +147,18 @@ class LearnedCorrector:
             self._bufs[key] = torch.empty((n, cols) if cols else tuple(shape), dtype=dtype, device=self.device)
         return self._bufs[key]
 
-    def pack(self, act, rho, dp):
+    def pack(self, act, rho, dp, scales=None):
         """BF16 network input; returns (x, max dp as a 1-element device tensor).
-        The scales are read on the device: no host synchronisation."""
+        The scales are read on the device: no host synchronisation.  ``scales``:
+        (max activity, max dp) device tensors to use instead of this input's own (a
+        slab of a larger volume is normalised by the whole volume's maxima)."""
         import torch
         x = self._buf(act.shape, "x", torch.bfloat16, 16)
-        amax = act.max().reshape(1)
-        dmax = dp.max().reshape(1)
+        if scales is None:
+            amax = act.max().reshape(1)
+            dmax = dp.max().reshape(1)
+        else:
+            amax, dmax = scales
         nat.check(self.lib.pvd_cnn_pack_inputs(ctypes.c_void_p(act.data_ptr()), ctypes.c_void_p(rho.data_ptr()),
                                                ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(amax.data_ptr()),
                                                ctypes.c_void_p(dmax.data_ptr()), ctypes.c_void_p(x.data_ptr()),
@@ -195,7 +202,37 @@ class LearnedCorrector:
                                                  self._stream()), "pvd_cnn_region_blocks")
         return blist, count
 
-    def correct(self, act, rho, r_override=None, region: bool = False) -> Correction:
+    def network_tiled(self, act, rho, dp, tile_z: int = 128, region: bool = False):
+        """The network's correction r (FP32, the volume's shape) computed slab by slab
+        along z: slabs of E voxels (E a multiple of 32, at most tile_z and Nz) that
+        overlap by 2 x 6 voxels, each keeping the outputs at least 6 voxels (the
+        receptive field of the six 3^3 layers) from its cut faces, so every output equals
+        the whole-volume network's.  Inputs are normalised by the whole volume's maxima.
+        Runs volumes whose Nz is not a multiple of 32, and bounds the activation
+        memory by the slab (16 + 5 x 32 BF16 channels per voxel)."""
+        import torch
+        shape = tuple(act.shape)
+        nz = shape[2]
+        if nz < 32:
+            raise ValueError(f"the network needs at least 32 voxels along z (got {nz})")
+        e = min(int(tile_z) // 32 * 32, nz // 32 * 32)
+        if e < 32:
+            raise ValueError("tile_z must be at least 32")
+        halo = 6
+        amax = act.max().reshape(1)
+        dmax = dp.max().reshape(1)
+        r = torch.empty(shape, dtype=torch.float32, device=self.device)
+        for s0, s1, a, b in z_slabs(nz, e, halo):
+            sl = (slice(None), slice(None), slice(s0, s1))
+            xs, _ = self.pack(act[sl].contiguous(), rho[sl].contiguous(), dp[sl].contiguous(), scales=(amax, dmax))
+            sshape = (shape[0], shape[1], s1 - s0)
+            blocks = self.region_blocks(dp[sl].contiguous(), dmax, self.region_frac) if region else None
+            rs = self.net.forward(xs, sshape, keep_activations=False, blocks=blocks).reshape(sshape)
+            r[:, :, a:b] = rs[:, :, a - s0:b - s0]
+        self._keep_scales = (amax, dmax)
+        return r.reshape(-1), dmax
+
+    def correct(self, act, rho, r_override=None, region: bool = False, tile_z=None) -> Correction:
         """CNN-corrected dose with fallback (one host synchronisation, for the
         fallback decision).  With ``region`` the network runs only on the blocks within
         its receptive field of the region dp >= region_frac * max dp, and the correction
@@ -209,7 +246,9 @@ class LearnedCorrector:
         shape = tuple(act.shape)
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
         apply_frac = self.region_frac if region else 0.0
-        if r_override is None:
+        if r_override is None and (tile_z is not None or shape[2] % 32):
+            r, dmax = self.network_tiled(act, rho, dp, tile_z or 128, region)
+        elif r_override is None:
             x, dmax = self.pack(act, rho, dp)
             blocks = self.region_blocks(dp, dmax, apply_frac) if region else None
             r = self.net.forward(x, shape, keep_activations=False, blocks=blocks)
@@ -231,11 +270,67 @@ class LearnedCorrector:
         return Correction(out, dp, r.reshape(shape), n_ex, fallback)
 
 
+def z_slabs(nz: int, e: int, halo: int = 6):
+    """Slabs [s0, s1) of extent e covering [0, nz) for the tiled network, with the range
+    [a, b) of each whose outputs are kept: at least `halo` voxels from a cut face (a
+    volume face needs no margin).  Consecutive kept ranges tile [0, nz) exactly."""
+    if e > nz or e <= 2 * halo:
+        raise ValueError("slab extent must lie in (2 * halo, nz]")
+    out = []
+    s0, a = 0, 0
+    while True:
+        if s0 + e > nz:
+            s0 = nz - e
+        s1 = s0 + e
+        b = nz if s1 == nz else s1 - halo
+        out.append((s0, s1, a, b))
+        if b == nz:
+            return out
+        a, s0 = b, b - halo
+
+
 def voxel_errors(dose, ref, region_frac: float = 0.1):
     """Relative voxel errors |D - D_ref| / D_ref on D_ref >= region_frac * max(D_ref)."""
     region = ref >= region_frac * ref.max()
     rel = (dose[region] - ref[region]).abs() / ref[region]
     return rel
+
+
+def subregion_box(act, size: int = 64):
+    """A size^3 box (slices) centred on the activity maximum, shifted inside the volume."""
+    import torch
+    shape = tuple(act.shape)
+    if min(shape) < size:
+        raise ValueError(f"volume {shape} is smaller than a {size}^3 box")
+    flat = int(torch.argmax(act).item())
+    c = np.unravel_index(flat, shape)
+    lo = [min(max(int(ci) - size // 2, 0), n - size) for ci, n in zip(c, shape)]
+    return tuple(slice(l, l + size) for l in lo)
+
+
+def evaluate_subregion(corr: LearnedCorrector, act, rho, box, histories: int = 4096, seed: int = 1,
+                       tol: float = 0.02, tile_z: int = 128):
+    """Plain and CNN-corrected dose against Monte Carlo on a sub-region of a large volume.
+
+    The activity is kept only inside `box`; the plain dose and the correction run on
+    the whole volume (the network in z slabs), and the Monte Carlo transports the box's
+    activity through the box's density.  All activity lies in the box, so inside it
+    both doses have the same sources (particles that leave the box and scatter back are
+    the only difference).  Errors on the box voxels >= 10 % of the Monte Carlo maximum."""
+    import torch
+    masked = torch.zeros_like(act)
+    masked[box] = act[box]
+    c = corr.correct(masked, rho, tile_z=tile_z)
+    ref = corr.mc_dose(masked[box].contiguous(), rho[box].contiguous(), histories, seed=seed)
+    plain, dose = c.plain[box], c.dose[box]
+    rp, rc = voxel_errors(plain, ref), voxel_errors(dose, ref)
+    out = {"plain_mean": float(rp.mean().item()), "cnn_mean": float(rc.mean().item()),
+           "plain_p90": float(torch.quantile(rp, 0.9).item()), "cnn_p90": float(torch.quantile(rc, 0.9).item()),
+           "plain_within": float((rp <= tol).float().mean().item()),
+           "cnn_within": float((rc <= tol).float().mean().item()),
+           "voxels": int(rp.numel()), "fallback": bool(c.fallback), "exceed": int(c.exceed),
+           "box": [[b.start, b.stop] for b in box], "histories": int(histories)}
+    return out
 
 
 def evaluate(corr: LearnedCorrector, seeds, shape=(128, 128, 128), histories: int = 128, tol: float = 0.02):

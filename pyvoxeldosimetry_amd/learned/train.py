@@ -90,7 +90,7 @@ def main(argv=None):
     ap.add_argument("--bound", type=float, default=0.5, help="|r| above which a voxel counts as out of bound")
     ap.add_argument("--no-augment", action="store_true")
     ap.add_argument("--val-every", type=int, default=0, help="validate on 8 held-out phantoms every N epochs")
-    ap.add_argument("--save", default="gpurun_out/cnn_y90_2mm.pt", help="weights + Adam state (torch.save)")
+    ap.add_argument("--save", default="cnn_weights.pt", help="weights + Adam state (torch.save)")
     ap.add_argument("--export", default=None, help="also write the FP32 weights as safetensors")
     ap.add_argument("--resume", default=None, help="continue from a --save file (weights and Adam state)")
     ap.add_argument("--init", default=None, help="start from these FP32 weights (safetensors), fresh Adam state")
@@ -98,13 +98,15 @@ def main(argv=None):
     ap.add_argument("--nuclide", default="Y90", choices=["Y90", "Lu177"])
     ap.add_argument("--voxel", type=float, default=2.0, help="voxel size in mm")
     ap.add_argument("--no-photons", action="store_true", help="electrons-only Monte Carlo")
+    ap.add_argument("--kernel-size", type=int, default=13, help="grid of the Monte Carlo point kernel of the plain dose")
     args = ap.parse_args(argv)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
     shape = (args.shape,) * 3
-    corr = LearnedCorrector(args.nuclide, voxel_mm=args.voxel, bound=args.bound, photons=not args.no_photons)
+    corr = LearnedCorrector(args.nuclide, voxel_mm=args.voxel, bound=args.bound, photons=not args.no_photons,
+                            kernel_size=args.kernel_size)
     if args.resume:
         corr.net.load_state_dict(torch.load(args.resume, weights_only=True))
         log(f"resumed from {args.resume} at Adam step {corr.net.step}")
@@ -146,6 +148,7 @@ def main(argv=None):
     if args.export:
         corr.net.save_weights(args.export, {"nuclide": args.nuclide, "voxel_mm": args.voxel, "shape": args.shape,
                                             "mc_photons": "0" if args.no_photons else "1",
+                                            "kernel_size": args.kernel_size,
                                             "command": "python -m pyvoxeldosimetry_amd.learned.train "
                                                        + " ".join(sys.argv[1:] if argv is None else argv),
                                             "train_seeds": f"{args.seed0}..{args.seed0 + args.n_train - 1}",

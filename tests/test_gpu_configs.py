@@ -10,7 +10,8 @@ rates, taken by linearity as one convolution of the trapezoid-weighted activity)
 Learned correction at 128^3: the residual CNN's forward on the GPU against the
 FP64 network oracle on windows (six 3^3 layers see 6 voxels, so a 20^3 input
 window pins its central 8^3 outputs, and a window at the volume edge also the
-outputs next to the edge), then the corrected dose with its fallback rule.
+outputs next to the edge), then the corrected dose with its fallback rule, and the
+network run in z slabs equal to the whole-volume network.
 """
 import sys
 
@@ -171,3 +172,33 @@ def test_region_restricted_network_equals_the_full_network_on_its_region(seed):
     assert torch.equal(part.r[reg], full_r[reg])
     assert torch.equal(part.dose[reg], full_dose[reg])
     assert torch.equal(part.dose[~reg], part.plain[~reg])
+
+
+def test_tiled_network_equals_the_whole_volume_network(cnn128):
+    """The network run in z slabs (extent 64, kept outputs >= 6 voxels from cut faces,
+    inputs normalised by the whole volume) gives the whole-volume network's output bit
+    for bit; on a 100-voxel-deep volume (not a multiple of 32, so only the slab path
+    runs) it equals the whole network on the 96-deep crop wherever the crop's own cut
+    face is more than 6 voxels away."""
+    import torch
+    corr, act, rho = cnn128
+    shape = (128, 128, 128)
+    _say("cnn tiled vs whole, 128^3")
+    dp = corr.plain(act, rho)
+    x, _ = corr.pack(act, rho, dp)
+    full = corr.net.forward(x, shape, keep_activations=False).clone().reshape(shape)
+    tiled, _ = corr.network_tiled(act, rho, dp, tile_z=64)
+    assert torch.equal(tiled.reshape(shape), full)
+    _say("cnn tiled, 128x128x100")
+    a2, r2 = act[:, :, :100].contiguous(), rho[:, :, :100].contiguous()
+    dp2 = corr.plain(a2, r2)
+    t2, _ = corr.network_tiled(a2, r2, dp2, tile_z=64)
+    t2 = t2.reshape(128, 128, 100)
+    assert torch.isfinite(t2).all()
+    # the whole network on the first 96 planes, normalised like the 100-deep volume
+    a3, r3, d3 = a2[:, :, :96].contiguous(), r2[:, :, :96].contiguous(), dp2[:, :, :96].contiguous()
+    x3, _ = corr.pack(a3, r3, d3, scales=(a2.max().reshape(1), dp2.max().reshape(1)))
+    f3 = corr.net.forward(x3, (128, 128, 96), keep_activations=False).reshape(128, 128, 96)
+    assert torch.equal(t2[:, :, :90], f3[:, :, :90])
+    c = corr.correct(a2, r2)
+    assert torch.isfinite(c.dose).all() and torch.equal(c.r, t2)

@@ -2,8 +2,8 @@
 
 256 x 2048 x 2048 (reference mode, 3^3 kernel): the workspace x-planes are
 2048 x 1040 complex apart, so the last planes lie more than 4 GiB past the first.
-The plan must route X through the register two-step instance for 256 (64-bit
-offsets) instead of the 256-point path; the check is the
+The plan must route X through the 64-bit generic pass (the register X passes
+address with 32-bit byte offsets); the check is the
 size-independent closed form of the reference convolution for a few deltas,
 D[(p + u) mod N] = a K[u] (kernel_convolution.py:71-74 with a kernel anchored at
 its origin), and zero elsewhere.
@@ -25,7 +25,7 @@ def test_deltas_at_256x2048x2048_land_where_the_closed_form_puts_them():
     print("[gpu-case] 256x2048x2048 deltas", file=sys.stderr, flush=True)
     k = dpk.lu177_kernel(2.0, (3, 3, 3))
     plan = ConvPlan(shape, k.shape, mode="reference", algo="fft")
-    assert plan.fft_n == shape and plan.fft_path[0] == "two_step"
+    assert plan.fft_n == shape and plan.fft_path[0] == "stockham"
     plan.set_kernel(k)
     act = torch.zeros(shape, dtype=torch.float32, device="cuda")
     deltas = [((253, 2040, 2000), 1.0), ((3, 100, 7), 0.5), ((130, 1500, 1024), 2.0), ((255, 2047, 2047), 1.5)]

@@ -28,11 +28,13 @@ def test_ct_and_torso_keep_their_register_paths():
     assert d["fft_path"] == ("two_step", "two_step", "two_step")
 
 
-def x_offsets_fit(nx, py, hp):
-    """The 32-bit byte offsets of the 256-point X pass: the x-plane walk and the
-    general kernel spectrum (restated from the kernel's addressing)."""
+def x_offsets_fit(nx, px, py, hp):
+    """The 32-bit byte offsets of the register X passes: the x-plane walk and the
+    general kernel spectrum, plus a tile's columns (restated from the kernels'
+    addressing)."""
     wplane = py * hp
-    return (nx - 1) * wplane * 8 <= 2 ** 32 - 1 - 256 and 255 * py * hp * 8 <= 2 ** 32 - 1 - 256
+    lim = 2 ** 32 - 1 - 256 - 8 * hp
+    return (nx - 1) * wplane * 8 <= lim and (px - 1) * py * hp * 8 <= lim
 
 
 @pytest.mark.parametrize("shape", [(256, 2048, 2048), (256, 2048, 1800), (256, 1600, 2048), (256, 2048, 1024),
@@ -43,10 +45,23 @@ def test_256_point_x_pass_only_where_32_bit_offsets_fit(shape):
     assert px == 256
     h = pz // 2 + 1
     hp = -(-h // 16) * 16
-    fits = x_offsets_fit(shape[0], py, hp)
+    fits = x_offsets_fit(shape[0], px, py, hp)
     assert (d["fft_path"][0] == "sq256") == fits, (shape, d["fft_path"], fits)
     if shape == (256, 2048, 2048):
         assert not fits                    # 255 * 2048 * 1040 * 8 bytes > 4 GiB
+
+
+@pytest.mark.parametrize("shape", [(384, 2048, 2048), (384, 1024, 1024), (320, 1600, 1800), (448, 512, 512)])
+def test_fused_two_step_x_pass_only_where_32_bit_offsets_fit(shape):
+    d = describe_plan(shape, (3, 3, 3), "reference", algo="fft")
+    px, py, pz = d["fft_n"]
+    assert px == shape[0] and px % 16 == 0 and px // 16 < 35
+    h = pz // 2 + 1
+    hp = -(-h // 16) * 16
+    fits = x_offsets_fit(shape[0], px, py, hp)
+    assert (d["fft_path"][0] == "two_step") == fits, (shape, d["fft_path"], fits)
+    if shape == (384, 2048, 2048):
+        assert not fits                    # 383 * 2048 * 1040 * 8 bytes > 4 GiB
 
 
 def test_register_padding_never_aliases_and_smooth_lengths_flag_restores_the_7_smooth_rule():

@@ -156,3 +156,21 @@ def test_plan_lengths_never_alias_and_paths_fit_their_lengths(shape, k, mode):
         if d["fft_path"][a] == "two_step":
             assert all(d["block"][i] <= 512 for i in pas)
     assert d["workspace_bytes"] >= 8 * shape[0] * d["fft_n"][1] * (d["fft_n"][2] // 2 + 1)
+
+
+# --------------------------------------------------------------------- tiled network slabs
+@settings(max_examples=200, deadline=None, derandomize=True)
+@given(nz=st.integers(32, 2000), k=st.integers(1, 16))
+def test_network_slabs_tile_the_axis_and_keep_the_receptive_field(nz, k):
+    """Slabs of extent e (a multiple of 32) whose kept ranges tile [0, nz) exactly and lie
+    at least 6 voxels from every cut face (the receptive field of six 3^3 layers)."""
+    from pyvoxeldosimetry_amd.learned.pipeline import z_slabs
+    e = min(32 * k, nz // 32 * 32)
+    sl = z_slabs(nz, e)
+    assert sl[0][2] == 0 and sl[-1][3] == nz
+    for (_, _, _, b), (_, _, c, _) in zip(sl, sl[1:]):
+        assert b == c
+    for s0, s1, a, b in sl:
+        assert s1 - s0 == e and 0 <= s0 <= a < b <= s1 <= nz
+        assert a == 0 or a - s0 >= 6
+        assert b == nz or s1 - b >= 6
