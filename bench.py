@@ -131,13 +131,15 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 256 MiB (reference for rocprof byte counters)")
-    ap.add_argument("--config", default="ct", choices=["ct", "batch", "torso", "tissue", "cnn", "tac", "resample"],
+    ap.add_argument("--config", default="ct",
+                    choices=["ct", "batch", "torso", "tissue", "cnn", "tac", "resample", "direct"],
                     help="ct: 256x256x200 single volume (default, the headline metric); batch: 48 uneven "
                          "phantoms x 4 timepoints in one graph; torso: 512x512x400, 61^3, 6 timepoints; "
                          "tissue: HU -> metal fill, class masks and density of a 512x512x400 CT; "
                          "cnn: CNN-corrected Y-90 dose of a 128^3 phantom (time, MFMA use, error vs Monte Carlo); "
                          "tac: voxelwise mono-exponential TAC fit + integral of 256x256x200 voxels x 4 timepoints; "
-                         "resample: trilinear 512x512x400 @0.98 mm -> 2 mm grid and back")
+                         "resample: trilinear 512x512x400 @0.98 mm -> 2 mm grid and back; "
+                         "direct: LDS direct convolution of 128^3 with a 13^3 kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -172,6 +174,8 @@ def main():
         return bench_tac(args)
     if args.config == "resample":
         return bench_resample(args)
+    if args.config == "direct":
+        return bench_direct(args)
 
     from pyvoxeldosimetry_amd import ConvPlan, Lu177KernelGenerator
     from pyvoxeldosimetry_amd.phantoms import ct_phantom
@@ -414,7 +418,43 @@ def bench_torso(args):
                                "sample": f"one reference np.fft conv of the 512x512x400 map ({one:.2f} s); the "
                                          f"reference repeats it per timepoint"}
         res["speedup_vs_cpu_reference"] = round(res["value"] / cpu_v, 1)
+    if not args.no_cnn:
+        res["cnn_corrected"] = torso_cnn(dev, acts, ph, rho, args)
     print(json.dumps(res), flush=True)
+
+
+def torso_cnn(dev, acts, ph, rho, args):
+    """Learned correction of the torso's time-integrated dose (Lu-177 network at 0.98 mm):
+    plain dose with the Monte Carlo point kernel, the network in z slabs of 128 (400 is
+    not a multiple of 32), bounded correction; its error against Monte Carlo on 64^3
+    sub-regions (activity kept only in the box): around the activity maximum, and across
+    the lower boundary of the right lung."""
+    import torch
+
+    from pyvoxeldosimetry_amd.core.activity_sampler import trapezoid_weights
+    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, evaluate_subregion, subregion_box
+    corr = LearnedCorrector.pretrained("Lu177", ph.voxel_mm)
+    w = [x * 3600.0 for x in trapezoid_weights(ph.time_points_h)]
+    cum = torch.zeros_like(acts[0])
+    for wi, a in zip(w, acts):
+        cum.add_(a, alpha=wi)
+    ms_plain = _timed(lambda: corr.plain(cum, rho), 5, 1)
+    ms_corr = _timed(lambda: corr.correct(cum, rho, tile_z=128), 3, 1)
+    c = corr.correct(cum, rho, tile_z=128)
+    out = {"ms_per_vol": round(ms_corr, 3), "plain_ms_per_vol": round(ms_plain, 3),
+           "ratio_corrected_over_plain": round(ms_corr / ms_plain, 2), "fallback": bool(c.fallback),
+           "workload": "torso 512x512x400 @0.98 mm, time-integrated activity of the 6 timepoints, Lu-177 "
+                       "Monte Carlo point kernel %d^3 (linear), 6-layer 3^3 residual CNN (BF16 MFMA) in z slabs of 128 "
+                       "(12-voxel overlaps), bounded correction" % corr.kernel.shape[0]}
+    if not args.no_eval:
+        sx, sy, sz = cum.shape
+        boxes = {"hottest_lesion": subregion_box(cum, 64),
+                 "lung_lower_edge": subregion_box(cum, 64, centre=(sx // 2, int(0.32 * sy), int(0.5 * sz)))}
+        out["error_vs_monte_carlo_subregion"] = {k: evaluate_subregion(corr, cum, rho, b, histories=4096)
+                                                 for k, b in boxes.items()}
+    return out
+
+
 
 
 def cnn_measure(dev, steps: int, warmup: int, eval_seeds=None, eval_shape=(64, 64, 64), histories: int = 8192):
@@ -547,6 +587,47 @@ def bench_tac(args):
         res["speedup_vs_cpu_reference"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
         pg = params[:, :m].cpu().numpy().astype(np.float64)
         res["max_rel_param_err_vs_curve_fit"] = float(np.max(np.abs(pg - pe) / np.abs(pe)))
+    print(json.dumps(res), flush=True)
+
+
+def bench_direct(args):
+    """The LDS-tiled direct convolution: 128^3 FP32 at 2 mm, Y-90 13^3 kernel (fixed centre),
+    reference semantics, density map; FP32 VALU roofline (2 FLOP per tap and voxel)."""
+    import numpy as np
+    import torch
+
+    from pyvoxeldosimetry_amd import ConvPlan, Y90KernelGenerator
+    from pyvoxeldosimetry_amd.learned.pipeline import synth_phantom
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    shape, grid = (128, 128, 128), (13, 13, 13)
+    act, rho = synth_phantom(31, shape, dev)
+    kern = Y90KernelGenerator("water", fix_centre=True).generate_kernel_device(2.0, grid, device=dev)
+    plan = ConvPlan(shape, grid, mode="reference", algo="direct", device=dev)
+    plan.set_kernel(kern)
+    out = torch.empty_like(act)
+    ms = _timed(lambda: plan.dose(act, density=rho, out=out), args.steps, args.warmup)
+    n = act.numel()
+    flops = 2.0 * float(np.prod(grid)) * n
+    # FP32 VALU peak without packed FP32 instructions (all HIP code here is built without them)
+    peak = 157.3e12 / 2
+    res = {"metric": "Gvoxel/s direct convolution 128^3, 13^3 kernel", "value": round(n / (ms * 1e-3) / 1e9, 3),
+           "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic (synth_phantom seed 31)",
+           "config": {"workload": "direct conv 128^3 f32 @2 mm, Y-90 13^3 kernel, density, reference semantics",
+                      "algo": plan.algo},
+           "roofline": {"bound": "valu", "achieved": round(flops / (ms * 1e-3) / 1e12, 2),
+                        "peak": round(peak / 1e12, 2), "unit": "TFLOP/s",
+                        "frac": round(flops / (ms * 1e-3) / peak, 4), "traffic": None,
+                        "kernel": "pvd_direct_kernel", "flop_per_step": flops}}
+    if not args.no_cpu:
+        from oracle import conv
+        got = out.cpu().numpy()
+        exp = conv.epilogue(conv.ref_conv(act.cpu().numpy().astype(np.float64), kern.cpu().numpy()),
+                            rho.cpu().numpy())
+        m = conv.error_metrics(got, exp)
+        res["error_vs_fp64"] = {"max_rel": m["max_rel"], "rms_rel": m["rms_rel"]}
     print(json.dumps(res), flush=True)
 
 

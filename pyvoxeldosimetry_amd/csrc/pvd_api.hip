@@ -741,15 +741,27 @@ int pvd_plan_set_kernel(pvd_plan *p, const double *kernel, void *stream) {
     }
     p->vol.sym = sym ? 1 : 0;
     const int H = p->P[2] / 2 + 1;
+    // FP64 scratch of the separable spectrum build: inside the plan's workspace when it
+    // fits (the workspace holds nothing between dose calls, and this call is ordered on
+    // `st` and synchronises before returning), else allocated for the call
     double2 *T1 = nullptr, *T2 = nullptr;
     const size_t n1 = (size_t)p->Ku[0] * p->Ku[1] * H, n2 = (size_t)p->Ku[0] * p->P[1] * H;
-    PVD_HIP(hipMalloc(&T1, sizeof(double2) * n1), "allocate spectrum scratch");
-    hipError_t e = hipMalloc(&T2, sizeof(double2) * n2);
-    if (e != hipSuccess) { (void)hipFree(T1); return hip_fail(e, "allocate spectrum scratch"); }
+    const bool in_work = sizeof(double2) * (n1 + n2) <= sizeof(float2) * p->work_elems;
+    hipError_t e = hipSuccess;
+    if (in_work) {
+        T1 = reinterpret_cast<double2 *>(p->vol.work);
+        T2 = T1 + n1;
+    } else {
+        PVD_HIP(hipMalloc(&T1, sizeof(double2) * n1), "allocate spectrum scratch");
+        e = hipMalloc(&T2, sizeof(double2) * n2);
+        if (e != hipSuccess) { (void)hipFree(T1); return hip_fail(e, "allocate spectrum scratch"); }
+    }
     e = pvd_launch_spectrum(p->P, p->lo, p->Ku, p->s, p->K, p->vol.Hp, sym ? 1 : 0, kernel, T1, T2, p->spec, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(T1);
-    (void)hipFree(T2);
+    if (!in_work) {
+        (void)hipFree(T1);
+        (void)hipFree(T2);
+    }
     if (e != hipSuccess) return hip_fail(e, "kernel spectrum");
     p->has_kernel = true;
     return PVD_OK;

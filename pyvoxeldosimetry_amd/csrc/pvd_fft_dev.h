@@ -459,7 +459,9 @@ __device__ void pass_y_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const float2 tw1 = ay.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
     const int rows_in = INV ? 256 : Ny;
-    const uint32_t lo = (uint32_t)(grp * Hp + l) * 8u, rstep = (uint32_t)Hp * 128u;   // + 16 rows
+    // lanes past the tile's last column (nc) read that column: no 64-byte sector of row
+    // padding is fetched (their values are masked; they store nothing)
+    const uint32_t lo = (uint32_t)(grp * Hp + min(l, nc - 1)) * 8u, rstep = (uint32_t)Hp * 128u;   // + 16 rows
     // every row r < 256 = Py and column c0 + l < Hp lies in the workspace: the loads
     // are unconditional (no branches around them) and masked afterwards
     float2 a[16];
@@ -505,13 +507,15 @@ __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     float2 *g = v.work + (long long)y * Hp + c0;       // uniform base; x-planes wplane apart
     const uint32_t pstride = (uint32_t)v.wplane * 8u;  // bytes per x-plane
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int lc = min(l, nc - 1);                     // lanes past nc read the last column
     const float2 tw1 = ax.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
-    // loads are unconditional (rows clamped to Nx - 1, columns c0 + l < Hp lie in the
-    // row padding) and masked afterwards, so no branch skips them
+    // loads are unconditional (rows clamped to Nx - 1, lanes past the tile's last column
+    // read that column, so no 64-byte sector of row padding is fetched) and masked
+    // afterwards, so no branch skips them
     float2 a[16];
 #pragma unroll
     for (int n1 = 0; n1 < 16; ++n1)
-        a[n1] = at((const float2 *)g, (uint32_t)min(16 * n1 + grp, Nx - 1) * pstride + (uint32_t)l * 8u);
+        a[n1] = at((const float2 *)g, (uint32_t)min(16 * n1 + grp, Nx - 1) * pstride + (uint32_t)lc * 8u);
     // kernel spectrum of rows k = grp + 16 k2, in flight through the forward transform;
     // columns l >= nc (row padding) multiply whatever the padding holds into columns
     // that are never stored, and no column mixes with another
@@ -523,7 +527,7 @@ __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const int k = grp + 16 * k2;
-            sp[k2] = make_float2(at(sr, (uint32_t)min(k, 256 - k) * krow + (uint32_t)l * 4u), 0.f);
+            sp[k2] = make_float2(at(sr, (uint32_t)min(k, 256 - k) * krow + (uint32_t)lc * 4u), 0.f);
         }
     } else {
         const float2 *sg = v.spec + (long long)y * Hp + c0;
@@ -531,7 +535,7 @@ __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const int k = grp + 16 * k2;
-            sp[k2] = at(sg, (uint32_t)k * kplane + (uint32_t)l * 8u);
+            sp[k2] = at(sg, (uint32_t)k * kplane + (uint32_t)lc * 8u);
         }
     }
     // all 33 loads are issued before any result is consumed
@@ -682,7 +686,7 @@ __device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl,
     const int TK = 1 << lnl;
     for (int w = threadIdx.x; w < (B << lnl); w += blockDim.x) {
         const int b = w >> lnl, l = w & (TK - 1);
-        const uint32_t lb = (uint32_t)l * 8u;
+        const uint32_t lb = (uint32_t)min(l, nc - 1) * 8u;     // past nc: the last column's sector
         float2 v[16];
 #pragma unroll
         for (int n1 = 0; n1 < 16; ++n1) v[n1] = at(g, (uint32_t)min(b + B * n1, rows_in - 1) * pitch_b + lb);

@@ -27,7 +27,7 @@ from .cnn import ResidualCNN
 from .mc import MonteCarlo
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
-PRETRAINED = {("Y90", 2.0): "cnn_y90_2mm.safetensors"}
+PRETRAINED = {("Y90", 2.0): "cnn_y90_2mm.safetensors", ("Lu177", 0.98): "cnn_lu177_0p98mm.safetensors"}
 EVAL_SEEDS_TEST = range(20000, 20006)      # held-out phantoms (training used seeds 10000..11999)
 
 
@@ -114,8 +114,9 @@ class LearnedCorrector:
         with safe_open(path, framework="pt") as f:
             file_meta = dict(f.metadata() or {})
         # the Monte Carlo the weights were trained against (files without the key: electrons
-        # only) and the grid of the point kernel of their plain dose (without the key: 13)
-        meta = dict(bound=0.9, photons=file_meta.get("mc_photons", "0") == "1",
+        # only), the grid of the point kernel of their plain dose (without the key: 13) and
+        # the correction bound chosen on held-out phantoms (without the key: 0.9)
+        meta = dict(bound=float(file_meta.get("bound", "0.9")), photons=file_meta.get("mc_photons", "0") == "1",
                     kernel_size=int(file_meta.get("kernel_size", "13")))
         meta.update(kw)
         c = cls(nuclide, voxel_mm=voxel_mm, **meta)
@@ -296,14 +297,14 @@ def voxel_errors(dose, ref, region_frac: float = 0.1):
     return rel
 
 
-def subregion_box(act, size: int = 64):
-    """A size^3 box (slices) centred on the activity maximum, shifted inside the volume."""
+def subregion_box(act, size: int = 64, centre=None):
+    """A size^3 box (slices) centred on `centre` (default: the activity maximum), shifted
+    inside the volume."""
     import torch
     shape = tuple(act.shape)
     if min(shape) < size:
         raise ValueError(f"volume {shape} is smaller than a {size}^3 box")
-    flat = int(torch.argmax(act).item())
-    c = np.unravel_index(flat, shape)
+    c = np.unravel_index(int(torch.argmax(act).item()), shape) if centre is None else centre
     lo = [min(max(int(ci) - size // 2, 0), n - size) for ci, n in zip(c, shape)]
     return tuple(slice(l, l + size) for l in lo)
 

@@ -206,3 +206,16 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         assert int(got[name]) == ctypes.sizeof(cls), name
         for f in fl:
             assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, (name, f)
+
+
+def test_subregion_box_centres_on_the_maximum_and_stays_inside():
+    import torch
+    from pyvoxeldosimetry_amd.learned.pipeline import subregion_box
+    a = torch.zeros((100, 80, 70))
+    a[50, 40, 35] = 1.0
+    assert subregion_box(a, 64) == (slice(18, 82), slice(8, 72), slice(3, 67))
+    a.zero_()
+    a[2, 79, 69] = 1.0                      # near faces: shifted inside
+    assert subregion_box(a, 64) == (slice(0, 64), slice(16, 80), slice(6, 70))
+    with pytest.raises(ValueError):
+        subregion_box(torch.zeros((32, 80, 70)), 64)
