@@ -185,7 +185,11 @@ def main():
     ph = ct_phantom(seed=2)
     kern = Lu177KernelGenerator("water").generate_kernel_device(ph.voxel_mm, ph.kernel_grid, device=dev)
     act = torch.from_numpy(ph.activity[0]).to(dev)
-    rho = torch.from_numpy(ph.density).to(dev)
+    # the density map from the phantom's synthetic CT through the HU -> density producer
+    from pyvoxeldosimetry_amd.tissue import density_from_hu
+    rho = density_from_hu(torch.from_numpy(ph.meta["hu"]).to(dev))
+    if not torch.equal(rho.cpu(), torch.from_numpy(ph.density)):
+        raise RuntimeError("HU -> density on the device differs from the phantom's table lookup")
     out = torch.empty_like(act)
     plan = ConvPlan(ph.shape, ph.kernel_grid, mode="reference", algo="auto", device=dev)
     plan.set_kernel(kern)
@@ -251,8 +255,9 @@ def main():
         "metric": METRIC, "value": round(gvox, 3), "unit": "Gvoxel/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded CT phantom, seed 2)",
-        "config": {"workload": "ct single volume 256x256x200 f32 @2.0 mm, Lu-177 31^3 kernel, density map, "
-                               "reference (circular) convolution", "algo": plan.algo,
+        "config": {"workload": "ct single volume 256x256x200 f32 @2.0 mm, Lu-177 31^3 kernel, density map from "
+                               "a synthetic CT (HU -> density producer), reference (circular) convolution",
+                   "algo": plan.algo,
                    "fft_n": list(plan.fft_n), "mode": "reference", "symmetric_spectrum": plan.symmetric_spectrum},
         "roofline": roof, "operator_roofline": operator_roof,
         "per_pass_ms": {k: round(v, 5) for k, v in acc.items()},

@@ -8,7 +8,9 @@ builders produce the benchmark configurations:
 
   small      64^3 at 4.42 mm, sphere r=10 at 1 MBq/mL + 5% Poisson noise, water
   ct         256x256x200 at 2.0 mm, liver ellipsoid 0.1 MBq/mL + 3 lesions
-             1-5 MBq/mL, density map with lung (0.3) and spine bone (1.85)
+             1-5 MBq/mL, synthetic CT in HU (soft tissue 61..69, lungs around
+             -700, spine 1200..1800) whose density map is the HU -> material ->
+             g/cc table of the tissue producer (Tissue 1.04, Lung 0.26, Bone 1.85)
   batch      48 phantoms, sizes U{96..384}^2 x U{96..320}, 4 timepoints with
              mono-exponential washout (T_eff 20-150 h), Y-90 or Lu-177
   torso      512x512x400 at 0.98 mm, torso with lungs/spine and 10 lesions,
@@ -101,8 +103,21 @@ def _body_density(shape, lungs, spine, soft=1.0, lung_rho=0.3, bone_rho=1.85, bo
     return rho
 
 
+def hu_to_density(hu) -> np.ndarray:
+    """g/cc of a CT in HU by the tissue producer's HU -> material table (material i
+    covers [lo_i, lo_{i+1}), HU_to_material.txt with GateMaterials.db densities): the
+    values pvd_tissue_map writes, computed on the host."""
+    from .tissue.composition import HU_MATERIALS
+    lo = np.array([m[0] for m in HU_MATERIALS], dtype=np.float64)
+    rho = np.array([m[2] for m in HU_MATERIALS], dtype=np.float32)
+    idx = np.clip(np.searchsorted(lo, np.asarray(hu, np.float64), side="right") - 1, 0, len(lo) - 1)
+    return rho[idx]
+
+
 def ct_phantom(seed: int = 2) -> Phantom:
-    """256x256x200 at 2 mm: liver 0.1 MBq/mL + 3 lesions 1-5 MBq/mL; lung/bone density."""
+    """256x256x200 at 2 mm: liver 0.1 MBq/mL + 3 lesions 1-5 MBq/mL; a synthetic CT in
+    HU (meta["hu"]) with lungs and a spine, and its density map (Tissue 1.04, Lung
+    0.26, Bone 1.85 g/cc by the HU -> material table)."""
     rng = np.random.default_rng(seed)
     shape = (256, 256, 200)
     v = 2.0
@@ -119,8 +134,18 @@ def ct_phantom(seed: int = 2) -> Phantom:
                 break
         _fill_sphere(act, c, r, bq_per_voxel(float(rng.uniform(1.0, 5.0)), v))
     lungs = [((128, 70, 160), (70, 40, 35)), ((128, 186, 160), (70, 40, 35))]
-    rho = _body_density(shape, lungs, ((200, 128), 12))
-    return Phantom("ct", v, [act], rho, [0.0], "Lu177", (31, 31, 31))
+    # HU kept inside each material's range of the table, so the density map stays
+    # piecewise constant: soft tissue [60, 80) -> Tissue, lungs [-949, -119) -> Lung,
+    # spine [1001, 2001) -> Bone
+    hu = rng.uniform(61.0, 69.0, size=shape).astype(np.float32)
+    for c, sz in lungs:
+        m = _ellipsoid(shape, c, sz)
+        hu[m] = np.clip(rng.normal(-700.0, 40.0, size=int(m.sum())), -900.0, -500.0)
+    x, y = np.ogrid[:shape[0], :shape[1]]
+    spine = (x - 200) ** 2 + (y - 128) ** 2 <= 12 ** 2
+    hu[spine, :] = rng.uniform(1200.0, 1800.0, size=(int(spine.sum()), shape[2]))
+    rho = hu_to_density(hu)
+    return Phantom("ct", v, [act], rho, [0.0], "Lu177", (31, 31, 31), {"hu": hu})
 
 
 @dataclass

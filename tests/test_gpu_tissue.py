@@ -116,3 +116,14 @@ def test_metal_fill_other_sigma_and_a_long_rod(shape):
     corr, _, _ = TissueComposition({"sigma": 0.6}).map(torch.from_numpy(ct).cuda())
     exp = ot.handle_artifacts(ct, sigma=0.6)
     np.testing.assert_allclose(corr.cpu().numpy(), exp, rtol=1e-6, atol=5e-4)
+
+
+def test_ct_phantom_density_is_its_hu_map_through_the_device_producer():
+    """The CT workload's density map: its synthetic CT through pvd_tissue_map on the GPU
+    equals the phantom's table lookup bit for bit."""
+    import torch
+    from pyvoxeldosimetry_amd.phantoms import ct_phantom
+    from pyvoxeldosimetry_amd.tissue import density_from_hu
+    p = ct_phantom()
+    rho = density_from_hu(torch.from_numpy(p.meta["hu"]).cuda())
+    assert torch.equal(rho.cpu(), torch.from_numpy(p.density))

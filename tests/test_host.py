@@ -135,11 +135,19 @@ def test_batch_specs_ranges():
         assert (s.nuclide, s.kernel_grid) in (("Y90", (11, 11, 11)), ("Lu177", (31, 31, 31)))
 
 
-def test_ct_phantom_density_classes():
+def test_ct_phantom_density_comes_from_its_hu_map():
+    """The CT phantom's density is its synthetic HU map through the HU -> material
+    table (checked against the oracle's restatement of the reference's table): soft
+    tissue 1.04, lung 0.26, bone 1.85 g/cc."""
+    from oracle import tissue as ot
     from pyvoxeldosimetry_amd.phantoms import ct_phantom
+    from pyvoxeldosimetry_amd.tissue.composition import HU_MATERIALS
     p = ct_phantom()
+    hu = p.meta["hu"]
+    assert hu.shape == p.shape and hu.dtype == np.float32
+    np.testing.assert_array_equal(p.density, ot.density_from_hu(hu, HU_MATERIALS).astype(np.float32))
     vals = set(np.unique(p.density).astype(np.float64).round(3).tolist())
-    assert vals == {0.3, 1.0, 1.85}
+    assert vals == {0.26, 1.04, 1.85}
     assert p.activity[0].max() > 10 * np.median(p.activity[0][p.activity[0] > 0])
 
 
