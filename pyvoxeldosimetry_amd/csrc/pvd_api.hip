@@ -208,6 +208,8 @@ struct pvd_batch {
     hipEvent_t uploaded = nullptr;     // last table upload (host table reusable after it)
     bool upload_pending = false;
     hipStream_t cap = nullptr;         // private stream used only for graph capture
+    hipStream_t fork[PVD_BATCH_BRANCHES]{};   // branch streams of the capture (default priority)
+    hipEvent_t join[PVD_BATCH_BRANCHES + 1]{};
     hipGraphExec_t exec = nullptr;
 };
 
@@ -349,14 +351,10 @@ int fft_layout(pvd_plan *p) {
     const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
     // y/x register paths: 1 = the 256-point path (16-line tiles, no reference-mode fold;
     // checked again in the kernel); 2 = the two-step P = 16 * rb (rb in kXyRegB) in its
-    // own kernel instances; the fused form (rb < PVD_XY_LDS_B) keeps 2 * foldn * TK
-    // elements of LDS past the tile for the fold (fewer lines per tile when that would
-    // pass the LDS budget)
+    // own kernel instances
     for (int a = 0; a < 2; ++a) {
         pvd_axis_d &ax = v.ax[a];
-        int &tk = a ? tky : tkx;
-        int &bl = a ? by : bx;
-        size_t &ld = a ? ldsy : ldsx;
+        const int tk = a ? tky : tkx;
         const bool sq = ax.n == 256 && ax.foldn == 0 && tk == 16;
         ax.reg = regs ? 1 : 0;
         ax.ra = ax.ra_log = ax.rb = 0;
@@ -365,10 +363,6 @@ int fft_layout(pvd_plan *p) {
             ax.ra = 16;
             ax.ra_log = 4;
             ax.rb = ax.n / 16;
-            const bool fused = ax.rb < PVD_XY_LDS_B;
-            auto fold = [&](int t) { return fused ? sizeof(float2) * 2 * (size_t)ax.foldn * (size_t)t : 0; };
-            while (tk > 2 && ld + fold(tk) > kFftLdsCap) pick_tile(ax.n, tk / 2, 0, &tk, &bl, &ld);
-            ld += fold(tk);
         }
     }
     // room for the padded exchange array of the register-resident 256-point path
@@ -831,6 +825,10 @@ int pvd_batch_destroy(pvd_batch *b) {
     if (b->uploaded) (void)hipEventSynchronize(b->uploaded);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (hipStream_t f : b->fork)
+        if (f) (void)hipStreamDestroy(f);
+    for (hipEvent_t j : b->join)
+        if (j) (void)hipEventDestroy(j);
     if (b->uploaded) (void)hipEventDestroy(b->uploaded);
     if (b->d_tab) (void)hipFree(b->d_tab);
     if (b->d_pool) (void)hipFree(b->d_pool);
@@ -961,6 +959,37 @@ int batch_upload(pvd_batch *b, hipStream_t st) {
     return PVD_OK;
 }
 
+// One pass's groups in the captured graph: branch k takes groups k, k + K, ...; the
+// groups of a pass run concurrently (each group's tail overlaps the others), and the
+// next pass starts when all of them are done.
+int batch_capture(pvd_batch *b) {
+    constexpr int K = PVD_BATCH_BRANCHES;
+    size_t i = 0;
+    while (i < b->launches.size()) {
+        const int pass = b->launches[i].pass;
+        PVD_HIP(hipEventRecord(b->join[K], b->cap), "record pass start");
+        for (int k = 0; k < K; ++k) PVD_HIP(hipStreamWaitEvent(b->fork[k], b->join[K], 0), "fork");
+        int k = 0;
+        for (; i < b->launches.size() && b->launches[i].pass == pass; ++i, k = (k + 1) % K) {
+            const batch_launch_d &L = b->launches[i];
+            pvd_batch_args a;
+            a.vols = b->d_tab;
+            a.base = b->d_pool + L.first;
+            a.vol = b->d_pool + b->pool_n + L.first;
+            a.nmem = L.nmem;
+            a.ntiles = L.cfg.grid;
+            PVD_HIP(pvd_launch_fft_pass_batch(L.pass, b->plans[L.exemplar]->vol, a, L.cfg.grid, L.cfg.block,
+                                              L.cfg.lds, b->fork[k]),
+                    "launch batch FFT pass");
+        }
+        for (int j = 0; j < K; ++j) {
+            PVD_HIP(hipEventRecord(b->join[j], b->fork[j]), "record branch end");
+            PVD_HIP(hipStreamWaitEvent(b->cap, b->join[j], 0), "join");
+        }
+    }
+    return PVD_OK;
+}
+
 int batch_launch(pvd_batch *b, hipStream_t st) {
     for (const batch_launch_d &L : b->launches) {
         pvd_batch_args a;
@@ -990,8 +1019,12 @@ int pvd_batch_run_graph(pvd_batch *b, void *stream) {
     if (!b->exec) {
         PVD_HIP(hipSetDevice(b->dev), "set device");
         if (!b->cap) PVD_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "create capture stream");
+        for (int k = 0; k < PVD_BATCH_BRANCHES; ++k)
+            if (!b->fork[k]) PVD_HIP(hipStreamCreateWithFlags(&b->fork[k], hipStreamNonBlocking), "create branch stream");
+        for (int k = 0; k <= PVD_BATCH_BRANCHES; ++k)
+            if (!b->join[k]) PVD_HIP(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming), "create event");
         PVD_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal), "begin capture");
-        int rc = batch_launch(b, b->cap);
+        int rc = batch_capture(b);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->cap, &g);
         if (rc) {

@@ -664,13 +664,42 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 // writes the rows back, and after one barrier the inverse DFT-16 of item (b, l)
 // leaves x[b + B n1] in registers, stored the same way.
 // Reference-mode fold: rows o < foldn also take index (o + N + off) mod P, which
-// another item holds.  Both operands go to a small LDS buffer past the tile, primary
-// rows at fb[o * TK + l] and folded rows at fb[(foldn + o) * TK + l], and after one
-// barrier row o gets primary + folded (the order of the LDS form of the pass).  The
-// host reserves 2 * foldn * TK elements for it.
+// another item holds, so a tile with a fold is written back to its own LDS rows (IY:
+// X[k] in row xy_row<B>(k); X: x[n] in row n) and stored by the row maps of
+// store_col_tile(_perm) after one barrier.  (A separate 2 * foldn * TK fold buffer
+// past the tile, for direct stores of the other rows, cost a workgroup per CU on the
+// 24-point tiles and 4 % of the 48-phantom batch.)
 // block bound of the two-step instances (their item loops are strided, so the host
 // caps the block at this; the register budget per instance is xy_waves())
 constexpr int kXyThreads = PVD_XY_MAX_BLOCK;
+
+// LDS row of FFT index k after the two-step: (k mod 16) B + k / 16
+template <int B>
+__device__ __forceinline__ int xy_row(int k) { return (k & 15) * B + (k >> 4); }
+
+// store_col_tile for a tile left in xy_row<B> order: FFT index k sits in LDS row
+// xy_row<B>(k) (same output mapping: row o takes index (o + off) mod P, plus index
+// (o + rows_out + off) mod P when o < foldn).
+template <int B>
+__device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long long pitch,
+                                                    const float2 *__restrict__ s, int rows_out, int lnl, int P,
+                                                    int off, int foldn, int nc) {
+    const int lp = lnl - 1;
+    const int total = rows_out << lp;
+    const int pmask = (1 << lp) - 1;
+    const float4 *s4 = reinterpret_cast<const float4 *>(s);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int o = e >> lp, cp = e & pmask;
+        if ((cp << 1) >= nc) continue;
+        float4 val = s4[(xy_row<B>(wrap_add(o, off, P)) << lp) + cp];
+        if (o < foldn) {
+            const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
+            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
+        }
+        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
+    }
+}
+
 
 // Addressing: every global access is a 32-bit byte offset from the tile's wave-uniform
 // base (one VGPR, no 64-bit address math: at()).  The host takes the fused form only
@@ -705,31 +734,12 @@ __device__ __forceinline__ void xy_step1_global(float2 *__restrict__ s, int lnl,
     }
 }
 
-// Output of FFT index j (column l of a tile, rows pitch_b bytes apart): row
-// (j - off) mod P when below rows_out, except the rows o < foldn and their folded
-// partners, which go to the fold buffer fb.
-__device__ __forceinline__ void xy_put(float2 *__restrict__ g, uint32_t pitch_b, float2 *__restrict__ fb, int lnl,
-                                       int j, int l, int nc, const pvd_axis_d &ax, float2 val) {
+// Output of FFT index j (column l of a tile, rows pitch_b bytes apart) without a
+// fold: row (j - off) mod P when below rows_out.
+__device__ __forceinline__ void xy_put(float2 *__restrict__ g, uint32_t pitch_b, int j, int l, int nc,
+                                       const pvd_axis_d &ax, float2 val) {
     const int o = j >= ax.off ? j - ax.off : j + ax.n - ax.off;
-    if (o < ax.N) {
-        if (o < ax.foldn) fb[(o << lnl) + l] = val;
-        else if (l < nc) at(g, (uint32_t)o * pitch_b + (uint32_t)l * 8u) = val;
-    } else if (o - ax.N < ax.foldn) {
-        fb[((ax.foldn + o - ax.N) << lnl) + l] = val;
-    }
-}
-
-// rows o < foldn: primary + folded, after the barrier that published fb
-__device__ __forceinline__ void xy_fold_store(float2 *__restrict__ g, uint32_t pitch_b,
-                                              const float2 *__restrict__ fb, int lnl, int nc, int foldn) {
-    const int TK = 1 << lnl;
-    for (int e = threadIdx.x; e < (foldn << lnl); e += blockDim.x) {
-        const int o = e >> lnl, l = e & (TK - 1);
-        if (l < nc) {
-            const float2 a = fb[e], f = fb[e + (foldn << lnl)];
-            at(g, (uint32_t)o * pitch_b + (uint32_t)l * 8u) = make_float2(a.x + f.x, a.y + f.y);
-        }
-    }
+    if (o < ax.N && l < nc) at(g, (uint32_t)o * pitch_b + (uint32_t)l * 8u) = val;
 }
 
 template <int B, bool INV>
@@ -743,7 +753,7 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nc = min(TK, H - c0);
     float2 *g = v.work + (long long)x * v.wplane + c0;    // uniform base of the tile
     const uint32_t pitch_b = (uint32_t)Hp * 8u;
-    float2 *fb = s + Py * TK;
+    const bool direct = !INV || ay.foldn == 0;           // block-uniform
     load_twiddles(tw, ay);
     lds_barrier();
     xy_step1_global<B, INV>(s, lnl, tw, g, pitch_b, INV ? Py : Ny, nc);
@@ -760,14 +770,19 @@ __device__ void pass_y_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 #pragma unroll
                 for (int k2 = 0; k2 < B; ++k2) at(g, o0 + (uint32_t)k2 * 16u * pitch_b) = u[k2];
             }
-        } else {
+        } else if (direct) {
 #pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) xy_put(g, pitch_b, fb, lnl, k1 + 16 * k2, l, nc, ay, u[k2]);
+            for (int k2 = 0; k2 < B; ++k2) xy_put(g, pitch_b, k1 + 16 * k2, l, nc, ay, u[k2]);
+        } else {
+            // rows k1 B .. k1 B + B-1 are read only by this item: X[k1 + 16 k2] back to
+            // row xy_row<B>(k1 + 16 k2) = k1 B + k2
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) s[(k1 * B + k2) * TK + l] = u[k2];
         }
     }
-    if (INV && ay.foldn) {
+    if (!direct) {
         lds_barrier();
-        xy_fold_store(g, pitch_b, fb, lnl, nc, ay.foldn);
+        store_col_tile_perm<B>(g, Hp, s, Ny, lnl, Py, ay.off, ay.foldn, nc);
     }
 }
 
@@ -789,7 +804,7 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const uint32_t pitch_b = (uint32_t)v.wplane * 8u;             // x-plane step of the workspace
     const uint32_t kplane_b = (uint32_t)Py * (uint32_t)Hp * 8u;   // kx step of the general spectrum
     const uint32_t krow_b = (uint32_t)(Py / 2 + 1) * (uint32_t)Hp * 4u;   // kx step of the real octant
-    float2 *fb = s + Px * TK;
+    const bool direct = ax.foldn == 0;                                  // block-uniform
     load_twiddles(tw, ax);
     lds_barrier();
     xy_step1_global<B, false>(s, lnl, tw, g, pitch_b, Nx, nc);
@@ -836,12 +851,18 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) u[k1] = s[(k1 * B + b) * TK + l];
         Dft<16, true>::run(u);
+        if (direct) {
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) xy_put(g, pitch_b, fb, lnl, b + B * n1, l, nc, ax, u[n1]);
+            for (int n1 = 0; n1 < 16; ++n1) xy_put(g, pitch_b, b + B * n1, l, nc, ax, u[n1]);
+        } else {
+            // rows b + B n1 are read only by this item: x[b + B n1] back to row b + B n1
+#pragma unroll
+            for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = u[n1];
+        }
     }
-    if (ax.foldn) {
+    if (!direct) {
         lds_barrier();
-        xy_fold_store(g, pitch_b, fb, lnl, nc, ax.foldn);
+        store_col_tile(g, (long long)v.wplane, s, Nx, lnl, Px, ax.off, ax.foldn, nc);
     }
 }
 
@@ -855,8 +876,6 @@ __device__ void pass_x_2s(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
 // xy_row<B>(k) = (k mod 16) B + k / 16.  xy_perm2nat: the same split run backwards,
 // from that order to natural order.
 constexpr int kXyLdsB = PVD_XY_LDS_B;
-template <int B>
-__device__ __forceinline__ int xy_row(int k) { return (k & 15) * B + (k >> 4); }
 template <int B, bool INV>
 __device__ __forceinline__ void xy_nat2perm(float2 *s, int lnl, const float2 *tw) {
     const int TK = 1 << lnl;
@@ -917,29 +936,6 @@ __device__ __forceinline__ void xy_perm2nat(float2 *s, int lnl, const float2 *tw
         for (int n1 = 0; n1 < 16; ++n1) s[(n1 * B + b) * TK + l] = v[n1];
     }
     lds_barrier();
-}
-
-// store_col_tile for a tile left in xy_row<B> order: FFT index k sits in LDS row
-// xy_row<B>(k) (same output mapping: row o takes index (o + off) mod P, plus index
-// (o + rows_out + off) mod P when o < foldn).
-template <int B>
-__device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long long pitch,
-                                                    const float2 *__restrict__ s, int rows_out, int lnl, int P,
-                                                    int off, int foldn, int nc) {
-    const int lp = lnl - 1;
-    const int total = rows_out << lp;
-    const int pmask = (1 << lp) - 1;
-    const float4 *s4 = reinterpret_cast<const float4 *>(s);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int o = e >> lp, cp = e & pmask;
-        if ((cp << 1) >= nc) continue;
-        float4 val = s4[(xy_row<B>(wrap_add(o, off, P)) << lp) + cp];
-        if (o < foldn) {
-            const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
-            val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
-        }
-        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
-    }
 }
 
 template <int B, bool INV>

@@ -321,10 +321,11 @@ constexpr int kZregB[PVD_ZREG_CODES] = {0, 16, 20, 24, 25, 16, 30, 20, 25, 27, 1
 #define PVD_XY_MAX_BLOCK 512
 // y/x lengths 16 * B other than 256 with a register two-step instance, grouped by
 // the source file that compiles them (pvd_fft_xy1/2/3.hip)
-// two-step lengths 16 * B with B >= PVD_XY_LDS_B run the LDS-resident form (no fold
-// buffer); smaller B the fused form, which keeps 2 * foldn * TK elements of LDS past
-// the tile for the reference-mode fold
+// two-step lengths 16 * B with B >= PVD_XY_LDS_B run the LDS-resident form (the tile
+// loaded into LDS by all threads); smaller B the form fused with its HBM loads
 #define PVD_XY_LDS_B 35
+// concurrent branches per pass in the batch graph (the groups of one pass overlap)
+#define PVD_BATCH_BRANCHES 4
 #define PVD_XY_BS_1(X) X(6) X(8) X(9) X(10) X(12) X(14) X(16)
 #define PVD_XY_BS_2(X) X(15) X(18) X(20) X(21) X(24) X(25)
 #define PVD_XY_BS_3(X) X(27) X(28) X(30) X(32) X(35) X(36)
