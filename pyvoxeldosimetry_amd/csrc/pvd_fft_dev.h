@@ -43,17 +43,31 @@ __device__ __forceinline__ int wrap_add(int a, int b, int n) {  // (a + b) mod n
 // The 256-point passes address at most Nx * wplane * 8 <= 256 * (256 * 1032 + pad) * 8
 // bytes of workspace and 129 * 129 * 1032 * 4 bytes of spectrum; an FZ tile spans at
 // most 2 * PVD_TILE_MAX lines of PVD_MAX_FFT_LEN floats: all well below 4 GiB.
+// Every pointer it takes is to global memory (HBM): the access is typed as such, so
+// that pointers read from a device-side descriptor table (the batch kernels) still
+// give global rather than flat loads and stores.
+// (The qualifier is for the device pass only: the host pass parses these inline
+// functions as well and has no global address space.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define PVD_GLOBAL __attribute__((address_space(1)))
+#else
+#define PVD_GLOBAL
+#endif
 template <typename T>
-__device__ __forceinline__ T &at(T *base, uint32_t byte_off) {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off);
+__device__ __forceinline__ PVD_GLOBAL T *gmem(T *p) {
+    return (PVD_GLOBAL T *)p;
 }
 template <typename T>
-__device__ __forceinline__ const T &at(const T *base, uint32_t byte_off) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+__device__ __forceinline__ PVD_GLOBAL T &at(T *base, uint32_t byte_off) {
+    return *(PVD_GLOBAL T *)((char *)base + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const PVD_GLOBAL T &at(const T *base, uint32_t byte_off) {
+    return *(const PVD_GLOBAL T *)((const char *)base + byte_off);
 }
 
 __device__ __forceinline__ void load_twiddles(float2 *tw, const pvd_axis_d &ax) {
-    for (int i = threadIdx.x; i < ax.n; i += blockDim.x) tw[i] = ax.tw[i];
+    for (int i = threadIdx.x; i < ax.n; i += blockDim.x) tw[i] = gmem(ax.tw)[i];
 }
 
 // Load rows [0, rows_in) x columns [0, nc) of a strided tile into LDS
@@ -73,7 +87,7 @@ __device__ __forceinline__ void load_col_tile(float2 *__restrict__ s, const floa
         for (int u = 0; u < kBatch; ++u) {
             const int e = e0 + u * blockDim.x;
             const int k = e >> lp, c = (e & pmask) << 1;
-            v[u] = (e < total && k < rows_in && c < nc) ? *reinterpret_cast<const float4 *>(g + k * pitch + c)
+            v[u] = (e < total && k < rows_in && c < nc) ? float4(*gmem(reinterpret_cast<const float4 *>(g + k * pitch + c)))
                                                         : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
@@ -108,7 +122,7 @@ __device__ __forceinline__ void store_col_tile(float2 *__restrict__ g, long long
             const float4 f = s4[(((o + rows_out + off) % P) << lp) + cp];
             val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
         }
-        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
+        *gmem(reinterpret_cast<float4 *>(g + o * pitch + (cp << 1))) = val;
     }
 }
 
@@ -357,8 +371,8 @@ __device__ __forceinline__ void fz_split(const pvd_vol_d &v, float2 *s, const lo
         const bool h1 = 2 * q + 1 < nreal;
         const float2 *pk = s + zrow(az, t) * ld + q;
         const float2 *pm = s + zrow(az, Pz - t) * ld + q;
-        float2 *g0 = v.work + lineoff[2 * q] + t;
-        float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0) + t;
+        PVD_GLOBAL float2 *g0 = gmem(v.work) + lineoff[2 * q] + t;
+        PVD_GLOBAL float2 *g1 = gmem(v.work) + (h1 ? lineoff[2 * q + 1] : 0) + t;
         for (int k0 = t, j0 = 0; k0 < H; k0 += TP * kZBatch, j0 += kZBatch) {
             float2 zk[kZBatch], zm[kZBatch];       // all LDS reads of the batch before any store
 #pragma unroll
@@ -457,7 +471,7 @@ __device__ void pass_y_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int nc = min(16, H - c0);
     float2 *g = v.work + (long long)x * v.wplane + c0;    // uniform base of the tile
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const float2 tw1 = ay.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
+    const float2 tw1 = gmem(ay.tw)[threadIdx.x];             // 256 twiddles, one per thread, in flight
     const int rows_in = INV ? 256 : Ny;
     // lanes past the tile's last column (nc) read that column: no 64-byte sector of row
     // padding is fetched (their values are masked; they store nothing)
@@ -508,7 +522,7 @@ __device__ void pass_x_sq(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const uint32_t pstride = (uint32_t)v.wplane * 8u;  // bytes per x-plane
     const int l = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const int lc = min(l, nc - 1);                     // lanes past nc read the last column
-    const float2 tw1 = ax.tw[threadIdx.x];             // 256 twiddles, one per thread, in flight
+    const float2 tw1 = gmem(ax.tw)[threadIdx.x];             // 256 twiddles, one per thread, in flight
     // loads are unconditional (rows clamped to Nx - 1, lanes past the tile's last column
     // read that column, so no 64-byte sector of row padding is fetched) and masked
     // afterwards, so no branch skips them
@@ -596,7 +610,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
-    const float2 *sp = v.spec + off0;
+    const PVD_GLOBAL float2 *sp = gmem(v.spec) + off0;
     load_twiddles(tw, ax);
     load_col_tile(s, g, plane, Px, Nx, lnl, nc);
     lds_barrier();
@@ -604,7 +618,7 @@ __device__ void pass_x(const pvd_vol_d &v, int tile, float2 *tw, float2 *s) {
     const int total = Px << lnl;
     if (v.sym) {
         // real spectrum, even in kx and ky: read the octant entry of (min(kx, Px-kx), min(y, Py-y))
-        const float *sr = reinterpret_cast<const float *>(v.spec);
+        const PVD_GLOBAL float *sr = (const PVD_GLOBAL float *)v.spec;
         const int Py = v.ax[1].n;
         const long long pyh = Py / 2 + 1;
         const long long row0 = (long long)min(y, Py - y) * Hp + c0;
@@ -696,7 +710,7 @@ __device__ __forceinline__ void store_col_tile_perm(float2 *__restrict__ g, long
             const float4 f = s4[(xy_row<B>((o + rows_out + off) % P) << lp) + cp];
             val.x += f.x; val.y += f.y; val.z += f.z; val.w += f.w;
         }
-        *reinterpret_cast<float4 *>(g + o * pitch + (cp << 1)) = val;
+        *gmem(reinterpret_cast<float4 *>(g + o * pitch + (cp << 1))) = val;
     }
 }
 
@@ -970,14 +984,14 @@ __device__ void pass_x_2s_lds(const pvd_vol_d &v, int tile, float2 *tw, float2 *
     const long long splane = (long long)v.ax[1].n * Hp;   // spectrum
     const long long off0 = (long long)y * Hp + c0;
     float2 *g = v.work + off0;
-    const float2 *sp = v.spec + off0;
+    const PVD_GLOBAL float2 *sp = gmem(v.spec) + off0;
     // After the forward two-step, LDS row r = k1 B + k2 holds frequency kx = k1 + 16 k2.
     // Every spectrum load is issued at a clamped address (the tile's first entry where
     // masked) and masked afterwards.  (Loading the first batch before the forward
     // transform took the 35-point instance from 131 to 170 VGPRs and X from 471 to
     // 735 us on the 512x512x400 torso: it halved the workgroups per CU.)
     const int total = Px << lnl;
-    const float *sr = reinterpret_cast<const float *>(v.spec);
+    const PVD_GLOBAL float *sr = (const PVD_GLOBAL float *)v.spec;
     const long long pyh = v.ax[1].n / 2 + 1;
     const long long row0 = (long long)min(y, v.ax[1].n - y) * Hp + c0;
     auto sym_index = [&](int e) -> long long {
@@ -1037,15 +1051,15 @@ __device__ __forceinline__ void iz_load(const pvd_vol_d &v, float2 *s, const lon
     for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {
         const int q = w >> tpl, t = w & (TP - 1);
         const bool h0 = 2 * q < nreal, h1 = 2 * q + 1 < nreal;
-        const float2 *g0 = v.work + (h0 ? lineoff[2 * q] : 0);
-        const float2 *g1 = v.work + (h1 ? lineoff[2 * q + 1] : 0);
+        const PVD_GLOBAL float2 *g0 = gmem(v.work) + (h0 ? lineoff[2 * q] : 0);
+        const PVD_GLOBAL float2 *g1 = gmem(v.work) + (h1 ? lineoff[2 * q + 1] : 0);
         for (int k0 = t; k0 < H; k0 += TP * kZBatch) {
             float2 A[kZBatch], B[kZBatch];
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
                 const int k = k0 + TP * u;
-                A[u] = (k < H && h0) ? g0[k] : make_float2(0.f, 0.f);
-                B[u] = (k < H && h1) ? g1[k] : make_float2(0.f, 0.f);
+                A[u] = (k < H && h0) ? float2(g0[k]) : make_float2(0.f, 0.f);
+                B[u] = (k < H && h1) ? float2(g1[k]) : make_float2(0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
@@ -1084,8 +1098,8 @@ __device__ __forceinline__ void iz_epilogue(const pvd_vol_d &v, const float2 *s,
 #pragma unroll
             for (int u = 0; u < kZBatch; ++u) {
                 const int o = oo + TP * u;
-                d[u].x = (has_dens && o < Nz) ? v.dens[o0 + o] : 1.f;
-                d[u].y = (has_dens && o < Nz && h1) ? v.dens[o0 + Nz + o] : 1.f;
+                d[u].x = (has_dens && o < Nz) ? gmem(v.dens)[o0 + o] : 1.f;
+                d[u].y = (has_dens && o < Nz && h1) ? gmem(v.dens)[o0 + Nz + o] : 1.f;
             }
             float2 vals[kZBatch];                   // all LDS reads of the batch before any store
 #pragma unroll
@@ -1108,8 +1122,8 @@ __device__ __forceinline__ void iz_epilogue(const pvd_vol_d &v, const float2 *s,
                         f0 *= density_factor(d[u].x);
                         f1 *= density_factor(d[u].y);
                     }
-                    v.out[o0 + o] = val.x * f0;
-                    if (h1) v.out[o0 + Nz + o] = val.y * f1;
+                    gmem(v.out)[o0 + o] = val.x * f0;
+                    if (h1) gmem(v.out)[o0 + Nz + o] = val.y * f1;
                 }
             }
         }
