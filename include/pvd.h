@@ -66,7 +66,8 @@ typedef struct pvd_plan_params {
     int fft_n[3];   /* FFT lengths per axis, 0 = plan's choice (7-smooth)       */
     int flags;      /* PVD_FLAG_*                                               */
     int tile[3];    /* lines per FFT tile: x pass, y pass (kz columns), z pass  */
-                    /* (complex line pairs); power of two <= 64, 0 = 16         */
+                    /* (complex line pairs); power of two <= 64, 0 = 16 (x: 8   */
+                    /* for the two-step lengths 288..512)                        */
     int max_radix;  /* largest power-of-two Stockham radix: 16 or 8, 0 = 16     */
 } pvd_plan_params;
 

@@ -346,9 +346,14 @@ int fft_layout(pvd_plan *p) {
     { int per = 1; while (per * lz < bz) per *= 2; bz = per * lz; }
     int tky, by; size_t ldsy;
     pick_tile(Py, p->tile[1], 0, &tky, &by, &ldsy);
-    int tkx, bx; size_t ldsx;
-    pick_tile(Px, p->tile[0], 0, &tkx, &bx, &ldsx);
     const bool regs = !(p->prm.flags & PVD_FLAG_STOCKHAM);
+    // x tile left to the plan: 8 columns for the fused two-step lengths 16 * B with
+    // B >= 18 (the X pass holds a column's two transforms and its spectrum values in
+    // registers there, and half-width tiles fit more workgroups per CU)
+    int tile_x = p->tile[0];
+    if (p->prm.tile[0] == 0 && regs && xy_two_step_len(Px) && Px >= 16 * 18 && Px < 16 * PVD_XY_LDS_B) tile_x = 8;
+    int tkx, bx; size_t ldsx;
+    pick_tile(Px, tile_x, 0, &tkx, &bx, &ldsx);
     // y/x register paths: 1 = the 256-point path (16-line tiles, no reference-mode fold;
     // checked again in the kernel); 2 = the two-step P = 16 * rb (rb in kXyRegB) in its
     // own kernel instances

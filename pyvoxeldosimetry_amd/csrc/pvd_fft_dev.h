@@ -1195,6 +1195,12 @@ __device__ __forceinline__ int x_band_tile() {
     return (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
 }
 
+// FY and IZ walk their tiles last to first: FZ and IY write the workspace in x-plane
+// order, so the planes they wrote last (those still in the last-level cache when the
+// workspace is larger than it) are read first.  (512x512x400 torso: FY 178 -> 167 us,
+// IZ 287 -> 272 us; the 256x256x200 CT dose 113.5 -> 112.3 us; same bits.)
+template <int PASS> constexpr bool kReversed = PASS == PVD_PASS_FY || PASS == PVD_PASS_IZ;
+
 // Batch launch: the member whose tile range holds grid tile t (base ascending),
 // then that volume's own tile.
 template <int PASS, bool RZ, int TP, int XB>
@@ -1204,6 +1210,7 @@ __device__ __forceinline__ void run_batch(const pvd_batch_args &a) {
         t = x_band_tile();
         if (t >= a.ntiles) return;
     }
+    if (kReversed<PASS>) t = a.ntiles - 1 - t;
     int lo = 0, hi = a.nmem - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -1244,6 +1251,7 @@ __global__ __launch_bounds__(1024) PVD_FFT_OCC void pvd_fft_pass(pvd_vol_d v) {
         tile = x_band_tile();
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
+    if (kReversed<PASS>) tile = v.tiles[PASS] - 1 - tile;
     run_pass<PASS, RZ, TP>(v, tile);
 }
 
@@ -1262,6 +1270,7 @@ void pvd_fft_pass_xy(pvd_vol_d v) {
         tile = x_band_tile();
         if (tile >= v.tiles[PVD_PASS_X]) return;
     }
+    if (kReversed<PASS>) tile = v.tiles[PASS] - 1 - tile;
     run_pass<PASS, false, 16, B>(v, tile);
 }
 

@@ -77,7 +77,8 @@ class ConvPlan:
                  algo: str = "auto", fft_n: Optional[Sequence[int]] = None, profile: bool = False,
                  device=None, general_spectrum: bool = False, tile: Optional[Sequence[int]] = None,
                  max_radix: int = 0, stockham: bool = False, smooth_lengths: bool = False):
-        """``tile``: lines per FFT tile of the (x, y, z) passes (powers of two, 0 = 16);
+        """``tile``: lines per FFT tile of the (x, y, z) passes (powers of two, 0 = 16, but 8
+        on x for the two-step lengths 288..512);
         ``max_radix``: largest power-of-two Stockham radix (16 or 8, 0 = 16);
         ``stockham``: LDS Stockham stages on every axis instead of the register-resident
         transforms (256-point y/x, two-step y/x/z);
