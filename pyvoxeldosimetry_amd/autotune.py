@@ -128,7 +128,7 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
             plan.close()
     if best is None:
         raise RuntimeError(f"no usable convolution plan for shape {shape}, kernel {kernel_shape}")
-    tile, radix, stockham = [16, 16, 16], 16, False
+    tile, radix, stockham = list(best[1].tile) if best[2] == "fft" else None, 16, False
     if best[2] == "fft" and search_tiles:
         fn = best[1].fft_n
 
@@ -141,8 +141,11 @@ def tuned_plan(shape, kernel_shape, mode, kernel_dev, max_candidates: int = 6, s
             p.set_kernel(kernel_dev)
             return time_plan(p), p
 
+        # from the tiles the plan chose itself (16 lines, 8 on x for some two-step lengths)
         for axis in (2, 1, 0):
-            for t in (8, 32):
+            for t in (8, 16, 32):
+                if t == tile[axis]:
+                    continue
                 cand = list(tile)
                 cand[axis] = t
                 res = trial(cand, radix)
