@@ -51,6 +51,24 @@ def test_256_point_x_pass_only_where_32_bit_offsets_fit(shape):
         assert not fits                    # 255 * 2048 * 1040 * 8 bytes > 4 GiB
 
 
+XY_TWO_STEP_B = [6, 8, 9, 10, 12, 14, 15, 16, 18, 20, 21, 24, 25, 27, 28, 30, 32, 35, 36]
+
+
+@pytest.mark.parametrize("b", XY_TWO_STEP_B)
+def test_plan_picks_8_column_x_tiles_for_the_fused_two_step_from_288_to_512_points(b):
+    """An x tile left to the plan is 8 columns for the fused two-step lengths 16 * B with
+    18 <= B <= 32 (the LDS-resident 560/576 and every shorter length keep 16); the y tile,
+    an explicit x tile and the Stockham flag are left alone."""
+    P = 16 * b
+    d = describe_plan((P, 6, 10), (5, 3, 3), "reference", algo="fft")
+    assert d["fft_n"][0] == P
+    assert d["fft_path"][0] == ("sq256" if P == 256 else "two_step")
+    assert d["tile"][0] == (8 if 18 <= b <= 32 else 16), d
+    assert describe_plan((6, P, 10), (3, 5, 3), "reference", algo="fft")["tile"][1] == 16
+    assert describe_plan((P, 6, 10), (5, 3, 3), "reference", algo="fft", tile=(16, 16, 16))["tile"][0] == 16
+    assert describe_plan((P, 6, 10), (5, 3, 3), "reference", algo="fft", stockham=True)["tile"][0] == 16
+
+
 @pytest.mark.parametrize("shape", [(384, 2048, 2048), (384, 1024, 1024), (320, 1600, 1800), (448, 512, 512)])
 def test_fused_two_step_x_pass_only_where_32_bit_offsets_fit(shape):
     d = describe_plan(shape, (3, 3, 3), "reference", algo="fft")
