@@ -478,10 +478,13 @@ void fill_desc(const pvd_plan *p, const float *const *acts, const float *w, int 
     v.dens = dens;
     v.out = out;
     v.scale = (float)scale;
-    // float4 access when every z-line starts 16-byte aligned
-    bool vec = p->N[2] % 4 == 0 && aligned16(out) && aligned16(dens);
-    for (int t = 0; t < T; ++t) vec = vec && aligned16(acts[t]);
-    v.vec_io = vec ? 1 : 0;
+    // float4 access when every z-line starts 16-byte aligned (1), else float4 runs of a
+    // tile's contiguous lines when the activity maps are 16-byte aligned and a tile's
+    // first line (a multiple of 2 * pairs per tile) starts a multiple of 4 floats in (2)
+    bool aligned = true;
+    for (int t = 0; t < T; ++t) aligned = aligned && aligned16(acts[t]);
+    const bool vec = aligned && p->N[2] % 4 == 0 && aligned16(out) && aligned16(dens);
+    v.vec_io = vec ? 1 : (aligned && p->vol.llz >= 1) ? 2 : 0;
 }
 
 int check_inputs(const pvd_plan *p, const float *const *acts, int T, const float *out, double scale) {

@@ -246,6 +246,81 @@ template <int TP> constexpr int z_tpl() { return TP == 16 ? 4 : TP == 32 ? 5 : 6
 // real line 2q in the real and 2q+1 in the imaginary part of complex line q.
 // FZ phases with TP threads per line pair (see kZBatch): the T-weighted load of
 // rows z < Nz, and the split of the two real spectra into the workspace.
+// FZ load of 16-byte aligned maps whose Nz is not a multiple of 4: the tile's real
+// lines L0 .. L0 + nreal - 1 are one contiguous run of nreal * Nz floats starting at a
+// multiple of 2 * nl lines (so 16-byte aligned), read as float4 (every load of a map
+// issued, at the run's first float4 where past its end, before any is used), summed
+// over the T maps, and each float written to its line pair's LDS row z (real part for
+// even lines, imaginary for odd).  The run's last nreal * Nz % 4 floats are read one by
+// one; lines past the last real line get zeros.
+__device__ __forceinline__ void fz_load_chunk(const pvd_vol_d &v, float2 *s, long long L0, int nreal, int nl, int ld,
+                                              int Nz) {
+    constexpr int kC4 = 8;                     // float4 per thread in flight per map
+    float *sf = reinterpret_cast<float *>(s);
+    const int n = nreal * Nz, n4 = n >> 2;
+    for (int e = threadIdx.x; e < (2 * nl - nreal) * Nz; e += blockDim.x) {
+        const int line = nreal + e / Nz, z = e - (e / Nz) * Nz;
+        sf[2 * (z * ld + (line >> 1)) + (line & 1)] = 0.f;
+    }
+    const float inv = 1.0f / (float)Nz;
+    auto put = [&](int f, float x) {           // float f of the run -> its LDS slot
+        int line = (int)((float)f * inv);
+        if (line * Nz > f) --line;
+        else if ((line + 1) * Nz <= f) ++line;
+        const int z = f - line * Nz;
+        sf[2 * (z * ld + (line >> 1)) + (line & 1)] = x;
+    };
+    for (int f0 = threadIdx.x; f0 < n4; f0 += kC4 * blockDim.x) {
+        float4 a[kC4];
+        auto load = [&](const float *map, float4 *x) {
+            const float4 *m = reinterpret_cast<const float4 *>(map + L0 * Nz);
+#pragma unroll
+            for (int u = 0; u < kC4; ++u) {
+                const int f = f0 + u * blockDim.x;
+                x[u] = at(m, (uint32_t)(f < n4 ? f : 0) * 16u);
+            }
+        };
+        load(v.act[0], a);
+        const float w0 = v.w[0];
+#pragma unroll
+        for (int u = 0; u < kC4; ++u) a[u] = make_float4(w0 * a[u].x, w0 * a[u].y, w0 * a[u].z, w0 * a[u].w);
+        for (int tt = 1; tt < v.T; ++tt) {
+            float4 x[kC4];
+            load(v.act[tt], x);
+            const float wt = v.w[tt];
+#pragma unroll
+            for (int u = 0; u < kC4; ++u) {
+                a[u].x = fmaf(wt, x[u].x, a[u].x); a[u].y = fmaf(wt, x[u].y, a[u].y);
+                a[u].z = fmaf(wt, x[u].z, a[u].z); a[u].w = fmaf(wt, x[u].w, a[u].w);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kC4; ++u) {
+            const int f = f0 + u * blockDim.x;
+            if (f < n4) {
+                // the four floats of float4 f: one division, then step along the line
+                const int e0 = 4 * f;
+                int line = (int)((float)e0 * inv);
+                if (line * Nz > e0) --line;
+                else if ((line + 1) * Nz <= e0) ++line;
+                int z = e0 - line * Nz;
+                const float x4[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    sf[2 * (z * ld + (line >> 1)) + (line & 1)] = x4[c];
+                    if (++z == Nz) { z = 0; ++line; }
+                }
+            }
+        }
+    }
+    for (int f = 4 * n4 + (int)threadIdx.x; f < n; f += blockDim.x) {
+        const float *m = v.act[0] + L0 * Nz;
+        float acc = v.w[0] * at(m, (uint32_t)f * 4u);
+        for (int tt = 1; tt < v.T; ++tt) acc = fmaf(v.w[tt], at(v.act[tt] + L0 * Nz, (uint32_t)f * 4u), acc);
+        put(f, acc);
+    }
+}
+
 template <int TP>
 __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long L0, int nreal, int nl, int ld,
                                         int Nz) {
@@ -254,7 +329,9 @@ __device__ __forceinline__ void fz_load(const pvd_vol_d &v, float2 *s, long long
     // Every load of a map is issued (at element 0 of the map where the line or index is
     // past the end) before any is used, and masked afterwards: loads under a branch are
     // waited for one by one.
-    if (v.vec_io) {            // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + TP j)
+    if (v.vec_io == 2) {
+        fz_load_chunk(v, s, L0, nreal, nl, ld, Nz);
+    } else if (v.vec_io) {     // Nz % 4 == 0 and 16-byte aligned maps: float4 z = 4 (t + TP j)
         const int nz4 = Nz >> 2;
         constexpr int kB4 = kZBatch / 2;
         for (int w = threadIdx.x; w < nl * TP; w += blockDim.x) {

@@ -60,7 +60,8 @@ struct pvd_vol_d {
     int llz;                          // log2 of complex line pairs per z-pass tile
     int ltky;                         // log2 of columns per y-pass tile
     int ltkx;                         // log2 of columns per x-pass tile
-    int vec_io;                       // 1: Nz % 4 == 0 and act/dens/out 16-byte aligned (float4 FZ loads)
+    int vec_io;                       // 1: Nz % 4 == 0 and act/dens/out 16-byte aligned (float4 FZ loads
+                                      // along the lines); 2: act 16-byte aligned (float4 runs of a tile)
     int sym;                          // 1: spec holds the real octant [Px/2+1][Py/2+1][Hp] (FP32)
     int wplane;                       // complex elements per x-plane of work (Py*Hp + padding)
     const float *act[PVD_MAX_TIMEPOINTS];
