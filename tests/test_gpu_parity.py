@@ -434,6 +434,45 @@ def test_xy_register_two_step_against_oracle_and_stockham(b):
         assert np.abs(outs[0] - outs[1]).max() <= 1e-6 * np.abs(exp).max()
 
 
+def _offset_copy(a, off):
+    """A contiguous float32 device copy of `a` starting `off` floats into its storage."""
+    import torch
+    buf = torch.empty(a.size + off, dtype=torch.float32, device="cuda")
+    view = buf[off:].view(a.shape)
+    view.copy_(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)))
+    return view
+
+
+@pytest.mark.parametrize("shape,tile", [((9, 7, 13), None), ((20, 11, 31), None), ((6, 5, 40), None),
+                                        ((9, 7, 13), (16, 16, 1))])
+def test_fz_load_forms_agree_bit_for_bit(shape, tile):
+    """FZ reads the T activity maps as float4 along each line (Nz % 4 == 0, 16-byte
+    aligned maps), as float4 runs of a tile's contiguous lines (aligned maps, any Nz,
+    tiles of >= 2 line pairs) or as 4-byte loads (maps not 16-byte aligned, or 1-pair
+    tiles).  The same maps, 16-byte aligned and one float off, give the FP64 oracle's
+    weighted dose and the same bits."""
+    import torch
+    from pyvoxeldosimetry_amd import ConvPlan
+    _say("fz load forms", shape, tile)
+    rng = np.random.default_rng(sum(shape))
+    acts = [rng.random(shape).astype(np.float32) for _ in range(3)]
+    w = [0.5, 2.0, 1.5]
+    k = dpk.lu177_kernel(2.0, (5, 7, 9))
+    plan = ConvPlan(shape, k.shape, mode="reference", algo="fft", tile=tile)
+    plan.set_kernel(k)
+    exp = conv.ref_conv(sum(wi * a.astype(np.float64) for wi, a in zip(w, acts)), k)
+    outs = []
+    for off in (0, 1):
+        maps = [_offset_copy(a, off) if off else dev_tensor(a) for a in acts]
+        assert all((m.data_ptr() % 16 == 0) == (off == 0) for m in maps)
+        got = plan.dose_weighted(maps, w).cpu().numpy()
+        assert_parity(got, exp, what=f"fz load {shape} tile {tile} offset {off}")
+        outs.append(got)
+    torch.cuda.synchronize()
+    plan.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_xy_two_step_linear_time_series_with_density():
     """Linear mode (no fold, offset 0) on a 16 * B length (y: 313 + 7 -> 320 = 16 x 20)
     with a 3-timepoint trapezoid and density: the time-integrated dose matches the FP64
