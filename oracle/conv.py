@@ -151,7 +151,7 @@ def ref_absorbed_dose(activity_maps, time_points, kernel, conv=ref_conv) -> np.n
     One dose-rate map per timepoint, then
     ``sum_i (D_i + D_{i+1})/2 * (t_{i+1}-t_i)*3600`` (hours -> seconds, no tail).
     Accumulated in FP64 (the reference's ``zeros_like`` follows the input dtype,
-    ``:94``; the oracle keeps FP64 as the judge).
+    ``:94``; the oracle keeps FP64, the precision every result is checked against).
     """
     rates = [conv(a, kernel) for a in activity_maps]
     dose = np.zeros(np.shape(activity_maps[0]), dtype=np.float64)
