@@ -208,6 +208,89 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
+// Output layer (32 channels in, channel 0 out, FP32, + bias[0]), by taps:
+//   Y[u][t] = sum_ci W[t][0][ci] x[u][ci]   for every voxel u of the block's halo, t < 27
+//   y[v]    = sum_t Y[v + tap(t)][t] + bias[0]
+// The first is a GEMM of the halo (32 voxels per MFMA row block, 32 channels) with the
+// 32 x 27 weight columns (taps padded to 32), so the matrix cores do 27 useful columns of
+// 32 instead of 1 of the 32 output channels a full layer computes; the second is a
+// 27-term sum per output voxel from LDS, in tap order.  Blocks and the region list are
+// those of cnn_conv_fwd (2 x 4 rows of 32 voxels along z).
+constexpr int OUT_YS = 28;                                     // floats per halo voxel of Y
+constexpr int OUT_HALO = HX * HY * HZ;                         // 816 voxels
+constexpr int OUT_GROUPS = (OUT_HALO + 31) / 32;               // 26 row blocks of 32
+constexpr size_t OUT_LDS = (size_t)OUT_HALO * OUT_YS * 4;
+
+__global__ __launch_bounds__(FWD_THREADS) void cnn_conv_out(ConvArgs a) {
+    extern __shared__ uint4 lds_u4[];
+    float *ys = reinterpret_cast<float *>(lds_u4);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 31, kh = lane >> 5;
+    constexpr int NW = FWD_THREADS / 64;
+    constexpr int GPW = (OUT_GROUPS + NW - 1) / NW;            // row blocks per wave (at most)
+    // B: column n = tap (zero past 26), rows = input channels; both K halves of 16
+    bf16x8 bw[2];
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (m < 27) v = *reinterpret_cast<const uint4 *>(a.w + ((size_t)m * 32 * 32 + cc * 16 + kh * 8));
+        bw[cc] = __builtin_bit_cast(bf16x8, v);
+    }
+    const float b0 = a.bias ? a.bias[0] : 0.0f;
+    const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
+    const int total = a.list ? *a.list_n : nbx * nby * nbz;
+    for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
+        const int b = a.list ? a.list[bi] : bi;
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        // A fragments of this wave's row blocks, all loads issued before the MFMAs
+        uint4 av[GPW][2];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int u = (wave + j * NW) * 32 + m;
+            const int hz = u % HZ, hy = (u / HZ) % HY, hx = u / (HZ * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            const bool in = u < OUT_HALO && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2;
+            const uint4 *src = reinterpret_cast<const uint4 *>(
+                a.x + (in ? (((long long)gx * a.n1 + gy) * a.n2 + gz) * 32 : 0));
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const uint4 v = src[cc * 2 + kh];
+                av[j][cc] = in ? v : make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        lds_barrier();      // the previous block's sums have read Y
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const int g = wave + j * NW;
+            if (g >= OUT_GROUPS) continue;           // wave-uniform
+            f32x16 acc = {};
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[j][0]), bw[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[j][1]), bw[1], acc, 0, 0, 0);
+            if (m < 27) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int u = g * 32 + mfma_row(i, lane);
+                    if (u < OUT_HALO) ys[u * OUT_YS + m] = acc[i];
+                }
+            }
+        }
+        lds_barrier();
+        if (tid < FWD_BX * FWD_BY * 32) {
+            const int ox = tid >> 7, oy = (tid >> 5) & 3, z = tid & 31;
+            if (x0 + ox < a.n0 && y0 + oy < a.n1) {
+                float sum = 0.0f;
+                for (int t = 0; t < 27; ++t) {
+                    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+                    sum += ys[(((ox + dx + 1) * HY + (oy + dy + 1)) * HZ + (z + dz + 1)) * OUT_YS + t];
+                }
+                a.y32[((long long)(x0 + ox) * a.n1 + (y0 + oy)) * a.n2 + z0 + z] = sum + b0;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
 struct WgradArgs {
     const uint16_t *x;
     const uint16_t *dz;
@@ -509,6 +592,18 @@ hipError_t launch_fwd(const ConvArgs &a, int blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_out(const ConvArgs &a, int blocks, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_out, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)OUT_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(cnn_conv_out, dim3(blocks), dim3(FWD_THREADS), OUT_LDS, st, a);
+    return hipGetLastError();
+}
+
 template <int CIN>
 hipError_t launch_wgrad(const WgradArgs &a, int blocks, hipStream_t st) {
     static bool attr = false;
@@ -575,7 +670,7 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
         switch (epi) {
             case 0: e = launch_fwd<32, 0>(a, grid, st); break;
             case 1: e = launch_fwd<32, 1>(a, grid, st); break;
-            case 2: e = launch_fwd<32, 2>(a, grid, st); break;
+            case 2: e = launch_out(a, grid, st); break;
             default: e = launch_fwd<32, 3>(a, grid, st); break;
         }
     }
