@@ -144,20 +144,36 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
         if (!live) continue;
         // two independent accumulation chains (by tap parity), so consecutive MFMAs
         // of a wave do not wait for each other's results
+        // The fragments of tap t + 1 are read before the MFMAs of tap t (two register
+        // sets), so each LDS round trip overlaps the previous tap's MFMAs; the MFMA
+        // sequence into each accumulator is the unpipelined loop's.
         f32x16 acc0 = {}, acc1 = {};
-        for (int tap = 0; tap < 27; ++tap) {
+        constexpr int KC = CIN / 16;
+        bf16x8 fa[2][KC], fb[2][KC];
+        auto read_tap = [&](int tap, bf16x8 *ta, bf16x8 *tb) {
             const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
             const char *hrow = hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * VS + kh * 16;
             const char *wrow = wl + (tap * 32 + m) * WS + kh * 16;
 #pragma unroll
-            for (int cc = 0; cc < CIN / 16; ++cc) {
-                if (((tap + cc) & 1) == 0)
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32),
-                                                                   acc0, 0, 0, 0);
-                else
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_frag(hrow + cc * 32), lds_frag(wrow + cc * 32),
-                                                                   acc1, 0, 0, 0);
+            for (int cc = 0; cc < KC; ++cc) {
+                ta[cc] = lds_frag(hrow + cc * 32);
+                tb[cc] = lds_frag(wrow + cc * 32);
             }
+        };
+        read_tap(0, fa[0], fb[0]);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            const int cur = tap & 1;
+            if (tap + 1 < 27) read_tap(tap + 1, fa[cur ^ 1], fb[cur ^ 1]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int cc = 0; cc < KC; ++cc) {
+                if (((tap + cc) & 1) == 0)
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[cur][cc], fb[cur][cc], acc0, 0, 0, 0);
+                else
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[cur][cc], fb[cur][cc], acc1, 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
         const f32x16 acc = acc0 + acc1;
         const int co = m;
