@@ -68,7 +68,9 @@ int pvd_mc_dose(const pvd_mc_params *p, const float *activity, const float *dens
  * Weights of a 3x3x3 layer are BF16 [27][32][cin] (tap-major, cout, cin
  * contiguous), tap = (dx+1)*9 + (dy+1)*3 + (dz+1); convolution is the
  * cross-correlation y[v] = sum_tap,ci x[v + d(tap)][ci] w[tap][co][ci] with zero
- * padding outside the grid.  cout is always 32; cin is 16 or 32. */
+ * padding outside the grid.  cout is always 32; cin is 16 or 32, or 4: a 16-channel
+ * input whose channels 4..15 are zero (only 0..3 are read, and channels 0..3 of the
+ * [27][32][16] weights), with the ReLU epilogue and no act (the input layer). */
 #define PVD_CNN_EPI_RELU        0  /* a = relu(z + b): writes a (bf16) to y and to act  */
 #define PVD_CNN_EPI_RESID_RELU  1  /* a = relu(z + b): act = a, y = resid + a (bf16)     */
 /* (act may be NULL in both: inference keeps no activations for a backward pass;

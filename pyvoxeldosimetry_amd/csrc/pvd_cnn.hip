@@ -224,6 +224,101 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
+// Input layer with four live channels (the packer's activity, density and plain dose,
+// and a zero) in a 16-channel BF16 input whose channels 4..15 are zero, ReLU epilogue.
+// K of each MFMA packs 4 taps x 4 channels: 7 MFMAs per 32-voxel row (taps 24..27, the
+// last zero) in place of 27 with 13 of 16 channels zero.  The packed weights
+// wp[c][co][k] = w[4c + k/4][co][k%4] are built in LDS from the [27][32][16] layout;
+// the halo holds the four channels (8 bytes a voxel).  Blocks, region lists, the
+// next-halo prefetch and the epilogue are cnn_conv_fwd's.
+constexpr int IN_CHUNKS = 7;
+constexpr int IN_WS = 16 * 2 + 16;                              // bytes per (chunk, cout) row
+constexpr int IN_HALO = HX * HY * HZ;
+constexpr size_t IN_LDS = (size_t)IN_CHUNKS * 32 * IN_WS + (size_t)IN_HALO * 8 + (size_t)(FWD_THREADS / 64) * FWD_STAGE;
+
+__global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
+    extern __shared__ uint4 lds_u4[];
+    char *lds = reinterpret_cast<char *>(lds_u4);
+    char *wl = lds;
+    char *hl = lds + IN_CHUNKS * 32 * IN_WS;
+    const int tid = threadIdx.x;
+    char *stg = hl + IN_HALO * 8 + (tid >> 6) * FWD_STAGE;
+    for (int i = tid; i < IN_CHUNKS * 32 * 16; i += FWD_THREADS) {
+        const int c = i >> 9, co = (i >> 4) & 31, k = i & 15;
+        const int tap = 4 * c + (k >> 2), ch = k & 3;
+        const uint16_t v = tap < 27 ? a.w[((size_t)tap * 32 + co) * 16 + ch] : (uint16_t)0;
+        *reinterpret_cast<uint16_t *>(wl + (c * 32 + co) * IN_WS + k * 2) = v;
+    }
+    const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
+    const int total = a.list ? *a.list_n : nbx * nby * nbz;
+    auto block_of = [&](int i) { return a.list ? a.list[i] : i; };
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wx = wave / FWD_BY, wy = wave % FWD_BY;
+    const int m = lane & 31, kh = lane >> 5;
+    const float bco = a.bias ? a.bias[m] : 0.0f;
+    constexpr int PER_T = (IN_HALO + FWD_THREADS - 1) / FWD_THREADS;
+    uint2 pf[PER_T];
+    auto fetch = [&](int b) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int vox = tid + u * FWD_THREADS;
+            const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            pf[u] = make_uint2(0u, 0u);
+            if (vox < IN_HALO && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
+                pf[u] = *reinterpret_cast<const uint2 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * 16);
+        }
+    };
+    if ((int)blockIdx.x < total) fetch(block_of(blockIdx.x));
+    for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
+        const int b = block_of(bi);
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        lds_barrier();     // weights visible / previous block finished with the halo
+#pragma unroll
+        for (int u = 0; u < PER_T; ++u) {
+            const int vox = tid + u * FWD_THREADS;
+            if (vox < IN_HALO) *reinterpret_cast<uint2 *>(hl + vox * 8) = pf[u];
+        }
+        lds_barrier();
+        const int ox = x0 + wx, oy = y0 + wy;
+        const bool live = ox < a.n0 && oy < a.n1;    // wave-uniform
+        const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
+        if (bi + (int)gridDim.x < total) fetch(block_of(bi + gridDim.x));
+        if (!live) continue;
+        f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+        for (int c = 0; c < IN_CHUNKS; ++c) {
+            uint2 h[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int tap = 4 * c + 2 * kh + j;
+                const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+                h[j] = tap < 27 ? *reinterpret_cast<const uint2 *>(
+                                      hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * 8)
+                                : make_uint2(0u, 0u);
+            }
+            const bf16x8 fa = __builtin_bit_cast(bf16x8, make_uint4(h[0].x, h[0].y, h[1].x, h[1].y));
+            const bf16x8 fb = lds_frag(wl + (c * 32 + m) * IN_WS + kh * 16);
+            if ((c & 1) == 0) acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc0, 0, 0, 0);
+            else acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc1, 0, 0, 0);
+        }
+        const f32x16 acc = acc0 + acc1;
+        // the output tile (32 voxels x 32 channels) staged in the wave's LDS slot as
+        // [row][channel], then written as two 16-byte stores per lane
+        uint16_t *sy = reinterpret_cast<uint16_t *>(stg);
+        const uint4 *src = reinterpret_cast<const uint4 *>(stg);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sy[mfma_row(i, lane) * 32 + m] = f2bf(fmaxf(acc[i] + bco, 0.0f));
+        uint4 *dst = reinterpret_cast<uint4 *>(a.y + vrow * 32);
+        dst[lane] = src[lane];
+        dst[lane + 64] = src[lane + 64];
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Output layer (32 channels in, channel 0 out, FP32, + bias[0]), by taps:
 //   Y[u][t] = sum_ci W[t][0][ci] x[u][ci]   for every voxel u of the block's halo, t < 27
 //   y[v]    = sum_t Y[v + tap(t)][t] + bias[0]
@@ -608,6 +703,18 @@ hipError_t launch_fwd(const ConvArgs &a, int blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_in(const ConvArgs &a, int blocks, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_in, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)IN_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(cnn_conv_in, dim3(blocks), dim3(FWD_THREADS), IN_LDS, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_out(const ConvArgs &a, int blocks, hipStream_t st) {
     static bool attr = false;
     if (!attr) {
@@ -655,7 +762,9 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
     if ((blocks == nullptr) != (nblocks == nullptr)) return pvd_set_error(PVD_EINVAL, "blocks and nblocks go together");
     if (int rc = check_grid(n0, n1, n2)) return rc;
     if (!x || !w) return pvd_set_error(PVD_EINVAL, "null input or weights");
-    if (cin != 16 && cin != 32) return pvd_set_error(PVD_EINVAL, "cin must be 16 or 32 (got %d)", cin);
+    if (cin != 4 && cin != 16 && cin != 32) return pvd_set_error(PVD_EINVAL, "cin must be 4, 16 or 32 (got %d)", cin);
+    if (cin == 4 && (epi != PVD_CNN_EPI_RELU || act))
+        return pvd_set_error(PVD_EINVAL, "cin 4 (the input layer) takes the ReLU epilogue without act");
     switch (epi) {
         case PVD_CNN_EPI_RELU: if (!y) return pvd_set_error(PVD_EINVAL, "null output"); break;
         case PVD_CNN_EPI_RESID_RELU:
@@ -675,7 +784,9 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
     const int grid = (int)(total < num_cus() ? total : num_cus());
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
-    if (cin == 16) {
+    if (cin == 4) {
+        e = launch_in(a, grid, st);
+    } else if (cin == 16) {
         switch (epi) {
             case 0: e = launch_fwd<16, 0>(a, grid, st); break;
             case 1: e = launch_fwd<16, 1>(a, grid, st); break;

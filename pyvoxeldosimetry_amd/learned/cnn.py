@@ -156,7 +156,8 @@ class ResidualCNN:
         every layer runs on those blocks only, and r is defined on the region they cover."""
         ws = self._workspace(shape)
         h = ws["h"]
-        self._conv(x, 16, 0, nat.PVD_CNN_EPI_RELU, shape, y=h[0], blocks=blocks)
+        # the input's channels 4..15 are zero (pvd_cnn_pack): the four-channel input layer
+        self._conv(x, 4, 0, nat.PVD_CNN_EPI_RELU, shape, y=h[0], blocks=blocks)
         for l in range(1, 5):
             self._conv(h[l - 1], 32, l, nat.PVD_CNN_EPI_RESID_RELU, shape, y=h[l],
                        act=ws["act"][l] if keep_activations else None, resid=h[l - 1], blocks=blocks)

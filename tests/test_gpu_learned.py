@@ -94,6 +94,41 @@ def test_conv_forward_all_epilogues(cin):
                                    rtol=8e-3, atol=2e-3)
 
 
+@pytest.mark.parametrize("shape", [SHAPE, (35, 37, 64)])
+def test_input_layer_with_four_live_channels(shape):
+    """cin = 4: a 16-channel input whose channels 4..15 are zero (the packer's layout),
+    K packed as 4 taps x 4 channels; the ReLU output matches the oracle and the 16-channel
+    kernel on the same input (up to FP32 summation order), and other epilogues or an
+    act output are rejected."""
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    _say("cnn input layer cin 4", shape)
+    lib = nat.load()
+    rng = np.random.default_rng(4)
+    n = int(np.prod(shape))
+    x = np.zeros(shape + (16,), np.float32)
+    x[..., :4] = ocnn.bf16_round(rng.normal(size=shape + (4,)).astype(np.float32))
+    w = ocnn.bf16_round(rng.normal(scale=0.1, size=(27, 32, 16)).astype(np.float32))
+    b = rng.normal(scale=0.1, size=32).astype(np.float32)
+    z = ocnn.conv3(x, w) + b
+    xt, wt, bt = bf(x).reshape(n, 16), bf(w), torch.from_numpy(b).cuda()
+    y4 = torch.empty((n, 32), dtype=torch.bfloat16, device="cuda")
+    y16 = torch.empty_like(y4)
+    _conv(lib, xt, 4, wt, bt, nat.PVD_CNN_EPI_RELU, shape, y=y4)
+    _conv(lib, xt, 16, wt, bt, nat.PVD_CNN_EPI_RELU, shape, y=y16)
+    got = y4.float().cpu().numpy().reshape(shape + (32,))
+    np.testing.assert_allclose(got, np.maximum(z, 0), rtol=8e-3, atol=1e-3)
+    # BF16 outputs: the two summation orders may round to neighbouring BF16 values
+    d = (y4.float() - y16.float()).abs().max().item()
+    assert d <= 8e-3 * max(1.0, y16.float().abs().max().item()), d
+    act = torch.empty_like(y4)
+    with pytest.raises(ValueError):
+        _conv(lib, xt, 4, wt, bt, nat.PVD_CNN_EPI_RELU, shape, y=y4, act=act)
+    r = torch.empty(n, dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        _conv(lib, xt, 4, wt, bt, nat.PVD_CNN_EPI_LINEAR_F32, shape, y32=r)
+
+
 @pytest.mark.parametrize("cin", [16, 32])
 def test_conv_forward_many_blocks_per_workgroup(cin):
     """A grid with more output blocks than workgroups (ragged in x and y): every
