@@ -491,7 +491,11 @@ def cnn_measure(dev, steps: int, warmup: int, eval_seeds=None, eval_shape=(64, 6
     ms_net = ev0.elapsed_time(ev1) / reps
     n = int(np.prod(shape))
     alg = 2.0 * 27 * (3 * 32 + 4 * 32 * 32 + 32 * 1) * n          # useful MACs x 2 (3 in, 32 hidden, 1 out)
-    executed = 2.0 * 27 * (16 * 32 + 4 * 32 * 32 + 32 * 32) * n    # as issued (16/32-channel MFMA tiles)
+    # as issued on the matrix cores: input layer 7 MFMAs (4 taps x 4 channels each) per 32
+    # voxels x 32 channels, residual layers 27 taps x 32 x 32, output layer 26 x 2 MFMAs of
+    # halo voxels x 32 tap columns per block of 256 voxels (SQ_INSTS_VALU_MFMA_MOPS_BF16
+    # gives the same per-voxel figures)
+    executed = (2.0 * 7 * 16 * 32 + 4 * 2.0 * 27 * 32 * 32 + 26 * 2 * 32768 / 256) * n
     ev = None if eval_seeds is None else evaluate(corr, eval_seeds, eval_shape, histories=histories)
     return {"corr": corr, "ms_plain": ms_plain, "ms_corr": ms_corr, "ms_net": ms_net, "alg_flops": alg,
             "executed_flops": executed, "eval": ev, "act": act, "ms_corr_region": ms_corr_region,
