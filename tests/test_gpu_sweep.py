@@ -9,7 +9,8 @@ direct path when every used extent is <= 16), a density map or none, and a
 scale.  The FP64 oracle is kernel_convolution.py:71-74 (reference mode) or the
 'same' linear convolution, then scale * rho_w / rho; tolerance max|D - D64| <=
 1e-5 max|D64|.  A second sweep puts random volumes through one batch and checks
-every member bit-exact against its own single-volume plan, eager and in the graph.
+every member bit-exact against its own single-volume plan, eager and in the graph.  A
+third draws volumes with x extents 260..500, whose x lengths take 8-column x tiles.
 """
 import sys
 
@@ -84,6 +85,33 @@ def test_random_shape_sweep_against_the_fp64_oracle():
         worst = max(worst, m["max_rel"])
         assert m["max_rel"] <= MAX_REL, (i, shape, kern.shape, mode, algo, fft_n, m)
     _say(f"sweep: worst max_rel {worst:.2e}")
+
+
+def test_random_long_x_volumes_with_8_column_x_tiles():
+    """x extents 260..500 put the x lengths in the fused two-step range 288..512, where the
+    plan picks 8-column x tiles: random kernels, modes, densities and scales against the
+    FP64 oracle."""
+    from pyvoxeldosimetry_amd import ConvPlan
+    rng = np.random.default_rng(4242)
+    n8 = 0
+    for i in range(12):
+        shape = (int(rng.integers(260, 501)), int(rng.integers(4, 60)), int(rng.integers(4, 60)))
+        kern = _kernel(rng, shape)
+        mode = "reference" if rng.random() < 0.6 else "linear"
+        act = rng.gamma(2.0, 1.0, size=shape) * (rng.random(shape) < 0.3)
+        dens = rng.uniform(0.3, 1.9, size=shape) if rng.random() < 0.5 else None
+        scale = float(rng.uniform(0.1, 10.0))
+        _say(f"long x {i}: {shape} kernel {kern.shape} {mode}")
+        plan = ConvPlan(shape, kern.shape, mode=mode, algo="fft")
+        plan.set_kernel(kern)
+        got = plan.dose(_dev(act), density=_dev(dens) if dens is not None else None,
+                        scale=scale).cpu().numpy().astype(np.float64)
+        n8 += plan.tile[0] == 8
+        plan.close()
+        raw = conv.ref_conv_threaded(act, kern, workers=16) if mode == "reference" else conv.linear_same(act, kern)
+        m = conv.error_metrics(got, conv.epilogue(raw, dens, scale))
+        assert np.isfinite(got).all() and m["max_rel"] <= MAX_REL, (i, shape, kern.shape, mode, m)
+    assert n8 >= 10, n8
 
 
 def test_random_batch_members_bit_exact_with_single_plans():
