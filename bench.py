@@ -277,7 +277,7 @@ def main():
     if not args.no_cnn:
         # the metric's second half: CNN-corrected ms/vol and its error against held-out Monte Carlo
         from pyvoxeldosimetry_amd.learned.pipeline import EVAL_SEEDS_TEST
-        m = cnn_measure(dev, 50, 5, list(EVAL_SEEDS_TEST)[:3], histories=4096)
+        m = cnn_measure(dev, 50, 5, list(EVAL_SEEDS_TEST), histories=8192)
         ev = m["eval"]
         result["cnn_corrected"] = {
             "ms_per_vol": round(m["ms_corr"], 4), "plain_ms_per_vol": round(m["ms_plain"], 4),
@@ -290,7 +290,7 @@ def main():
                         "(BF16 MFMA) over the whole volume + bounded correction (region mode: only the blocks "
                         "within the receptive field of plain dose >= 5 % of max)",
             "error_vs_monte_carlo": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ev.items()},
-            "error_sample": "3 held-out phantoms (seeds 20000..20002) at 64^3, Monte Carlo 4096 histories/voxel; "
+            "error_sample": "6 held-out phantoms (seeds 20000..20005) at 64^3, Monte Carlo 8192 histories/voxel; "
                             "voxels >= 10 % of max"}
     print(json.dumps(result), flush=True)
 
