@@ -136,6 +136,12 @@ int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const
                              float apply_frac, float *out, unsigned long long *counts, long long n_voxels,
                              void *stream);
 
+/* Whole-volume fallback on the device, after pvd_cnn_apply_correction on the same stream:
+ * out = dp when counts[1] > 0 and counts[0] > max_exceed_frac * counts[1], else out is left
+ * as it is.  The decision needs no host synchronisation. */
+int pvd_cnn_apply_fallback(const float *dp, const unsigned long long *counts, double max_exceed_frac, float *out,
+                           long long n_voxels, void *stream);
+
 /* Sum rows of a [rows][cols] FP32 array in fixed order into out[cols] (+= if accumulate). */
 int pvd_reduce_rows(const float *part, int rows, int cols, float *out, int accumulate, void *stream);
 

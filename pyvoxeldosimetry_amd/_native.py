@@ -103,6 +103,7 @@ SIGNATURES = {
     "pvd_cnn_pack_inputs": (_I, [_P, _P, _P, _P, _P, _P, ctypes.c_longlong, _P]),
     "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, _P, ctypes.c_float, ctypes.c_float, _P, _P,
                                       ctypes.c_longlong, _P]),
+    "pvd_cnn_apply_fallback": (_I, [_P, _P, _D, _P, ctypes.c_longlong, _P]),
     # include/pvd_tissue.h
     "pvd_tissue_map": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
     # include/pvd_resample.h

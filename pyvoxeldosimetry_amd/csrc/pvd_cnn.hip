@@ -643,6 +643,17 @@ __global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, c
     }
 }
 
+// Whole-volume fallback decided on the device from cnn_apply's counts: out = dp when
+// counts[1] > 0 and counts[0] > max_exceed_frac * counts[1]; otherwise every block
+// returns at once.
+__global__ __launch_bounds__(256) void cnn_fallback(const float *__restrict__ dp,
+                                                    const unsigned long long *__restrict__ counts,
+                                                    double max_exceed_frac, float *__restrict__ out, long long n) {
+    const unsigned long long ex = counts[0], reg = counts[1];
+    if (!(reg > 0 && (double)ex > max_exceed_frac * (double)reg)) return;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) out[v] = dp[v];
+}
+
 // Region blocks of the forward kernel (FWD_BX x FWD_BY x 32 output voxels):
 // flag[b] = 1 if block b holds a voxel with dp >= frac * max; count reset.
 __global__ __launch_bounds__(256) void cnn_block_flags(const float *__restrict__ dp, int n0, int n1, int n2,
@@ -888,6 +899,18 @@ int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const
                        region_frac, apply_frac, out, counts, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch correction");
+}
+
+int pvd_cnn_apply_fallback(const float *dp, const unsigned long long *counts, double max_exceed_frac, float *out,
+                           long long n, void *stream) {
+    if (!dp || !counts || !out || n < 1 || !(max_exceed_frac >= 0.0))
+        return pvd_set_error(PVD_EINVAL, "bad argument");
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(cnn_fallback, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, counts,
+                       max_exceed_frac, out, n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch fallback");
 }
 
 int pvd_cnn_region_blocks(const float *dp, int n0, int n1, int n2, const float *dose_max, float frac, int *flags,

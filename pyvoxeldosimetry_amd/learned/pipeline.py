@@ -17,7 +17,6 @@ import ctypes
 import math
 import os
 import time
-from dataclasses import dataclass
 
 import numpy as np
 
@@ -66,13 +65,30 @@ def synth_phantom(seed: int, shape=(128, 128, 128), device=None):
     return act.contiguous(), rho.contiguous()
 
 
-@dataclass
 class Correction:
-    dose: object            # corrected dose (FP32 tensor)
-    plain: object           # plain-convolution dose
-    r: object               # predicted relative correction
-    exceed: int             # voxels (>= 10 % max) whose |r| exceeded the bound
-    fallback: bool          # whole volume fell back to the plain dose
+    """Result of `LearnedCorrector.correct`: ``dose`` (corrected, FP32), ``plain`` (the
+    plain-convolution dose), ``r`` (predicted relative correction).  ``exceed`` (voxels
+    >= 10 % of max whose |r| passed the bound) and ``fallback`` (the whole volume fell
+    back to the plain dose) read the device counts on first access, one host
+    synchronisation; the fallback itself was applied on the device."""
+
+    def __init__(self, dose, plain, r, counts, max_exceed_frac: float):
+        self.dose, self.plain, self.r = dose, plain, r
+        self._counts, self._frac, self._host = counts, max_exceed_frac, None
+
+    def _read(self):
+        if self._host is None:
+            self._host = tuple(int(v) for v in self._counts.tolist())
+        return self._host
+
+    @property
+    def exceed(self) -> int:
+        return self._read()[0]
+
+    @property
+    def fallback(self) -> bool:
+        n_ex, n_region = self._read()
+        return n_region > 0 and n_ex > self._frac * n_region
 
 
 class LearnedCorrector:
@@ -234,8 +250,8 @@ class LearnedCorrector:
         return r.reshape(-1), dmax
 
     def correct(self, act, rho, r_override=None, region: bool = False, tile_z=None) -> Correction:
-        """CNN-corrected dose with fallback (one host synchronisation, for the
-        fallback decision).  With ``region`` the network runs only on the blocks within
+        """CNN-corrected dose with fallback (decided on the device: no host
+        synchronisation).  With ``region`` the network runs only on the blocks within
         its receptive field of the region dp >= region_frac * max dp, and the correction
         applies there (the plain dose elsewhere; exact on that region).  The default
         runs it on the whole volume: the >= 10 %-max region of the Monte Carlo dose is
@@ -264,11 +280,12 @@ class LearnedCorrector:
                                                     ctypes.c_void_p(out.data_ptr()),
                                                     ctypes.c_void_p(counts.data_ptr()), dp.numel(), self._stream()),
                   "pvd_cnn_apply_correction")
-        n_ex, n_region = (int(v) for v in counts.tolist())
-        fallback = n_region > 0 and n_ex > self.max_exceed_frac * n_region
-        if fallback:
-            out.copy_(dp)
-        return Correction(out, dp, r.reshape(shape), n_ex, fallback)
+        # the whole-volume fallback is decided and applied on the device: no host sync
+        nat.check(self.lib.pvd_cnn_apply_fallback(ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(counts.data_ptr()),
+                                                  ctypes.c_double(self.max_exceed_frac),
+                                                  ctypes.c_void_p(out.data_ptr()), dp.numel(), self._stream()),
+                  "pvd_cnn_apply_fallback")
+        return Correction(out, dp, r.reshape(shape), counts, self.max_exceed_frac)
 
 
 def z_slabs(nz: int, e: int, halo: int = 6):

@@ -428,6 +428,30 @@ def test_correction_fallback_rule():
     torch.testing.assert_close(c.dose, dp)
 
 
+def test_device_fallback_decision_at_its_boundary():
+    """pvd_cnn_apply_fallback copies the plain dose exactly when exceed > frac * region
+    (the host rule of Correction.fallback), with no host synchronisation."""
+    import ctypes
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    from pyvoxeldosimetry_amd.learned.pipeline import Correction
+    _say("device fallback")
+    lib = nat.load()
+    n = 70001                                               # not a multiple of the block size
+    dp = torch.rand(n, device="cuda") + 1.0
+    for ex, reg, frac in [(5, 100, 0.05), (6, 100, 0.05), (0, 0, 0.05), (1, 0, 0.05), (0, 7, 0.0), (1, 7, 0.0)]:
+        counts = torch.tensor([ex, reg], dtype=torch.int64, device="cuda")
+        out = torch.full((n,), -1.0, device="cuda")
+        nat.check(lib.pvd_cnn_apply_fallback(ctypes.c_void_p(dp.data_ptr()), ctypes.c_void_p(counts.data_ptr()),
+                                             ctypes.c_double(frac), ctypes.c_void_p(out.data_ptr()), n, None))
+        expect = Correction(out, dp, None, counts, frac).fallback
+        assert expect == (reg > 0 and ex > frac * reg)
+        if expect:
+            assert torch.equal(out, dp), (ex, reg, frac)
+        else:
+            assert bool((out == -1.0).all()), (ex, reg, frac)
+
+
 def test_short_training_fits_its_phantoms():
     """A short run of the GPU training loop (forward, loss, backward, Adam, all on
     the HIP kernels) lowers the loss and the error against its own Monte Carlo targets."""
