@@ -475,8 +475,8 @@ def cnn_measure(dev, steps: int, warmup: int, eval_seeds=None, eval_shape=(64, 6
     shape = (128, 128, 128)
     act, rho = synth_phantom(123, shape, dev)
     ms_plain = _timed(lambda: corr.plain(act, rho), steps, warmup)
-    ms_corr = _timed(lambda: corr.correct(act, rho), max(1, steps // 5), warmup)
-    ms_corr_region = _timed(lambda: corr.correct(act, rho, region=True), max(1, steps // 5), warmup)
+    ms_corr = _timed(lambda: corr.correct(act, rho), steps, warmup)
+    ms_corr_region = _timed(lambda: corr.correct(act, rho, region=True), steps, warmup)
     dp = corr.plain(act, rho)
     region_blocks = corr.region_blocks(dp, dp.max().reshape(1), corr.region_frac)[1].item()
     all_blocks = int(corr.lib.pvd_cnn_region_block_count(*shape))
