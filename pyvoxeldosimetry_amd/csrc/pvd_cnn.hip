@@ -893,8 +893,10 @@ int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const
     if (!dp || !r || !dose_max || !out || !counts || n < 1 || !(bound >= 0.0f) || !(region_frac >= 0.0f) ||
         !(apply_frac >= 0.0f))
         return pvd_set_error(PVD_EINVAL, "bad argument");
-    long long blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    // 8 or more voxels per thread: the block's reduction and atomics are paid once per
+    // 2048 voxels rather than per 256
+    long long blocks = (n + 2047) / 2048;
+    if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(cnn_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, r, bound, dose_max,
                        region_frac, apply_frac, out, counts, n);
     const hipError_t e = hipGetLastError();
@@ -906,7 +908,7 @@ int pvd_cnn_apply_fallback(const float *dp, const unsigned long long *counts, do
     if (!dp || !counts || !out || n < 1 || !(max_exceed_frac >= 0.0))
         return pvd_set_error(PVD_EINVAL, "bad argument");
     long long blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 1024) blocks = 1024;     // blocks that only read the counts, most often
     hipLaunchKernelGGL(cnn_fallback, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dp, counts,
                        max_exceed_frac, out, n);
     const hipError_t e = hipGetLastError();
