@@ -128,6 +128,13 @@ int pvd_cnn_adam(float *param, const float *grad, float *m, float *v, void *para
 int pvd_cnn_pack_inputs(const float *activity, const float *density, const float *dose_plain, const float *act_max,
                         const float *dose_max, void *x, long long n_voxels, void *stream);
 
+/* The network input's scales in one pass: maxima[0] = max(activity), maxima[1] =
+ * max(dose_plain), for pvd_cnn_pack_inputs' act_max = maxima and dose_max = maxima + 1.
+ * maxima holds PVD_CNN_MAXIMA_FLOATS floats; those past the first two are scratch. */
+#define PVD_CNN_MAXIMA_FLOATS (2 + 2 * 1024)
+int pvd_cnn_input_maxima(const float *activity, const float *dose_plain, long long n_voxels, float *maxima,
+                         void *stream);
+
 /* Corrected dose with fallback: out = dp*(1+r) where dp >= apply_frac * (*dose_max) and
  * |r| <= bound, else dp (r is read only where applied: 0 = everywhere).
  * counts[0] += voxels with |r| > bound and dp >= region_frac * (*dose_max);

@@ -104,6 +104,7 @@ SIGNATURES = {
     "pvd_cnn_apply_correction": (_I, [_P, _P, ctypes.c_float, _P, ctypes.c_float, ctypes.c_float, _P, _P,
                                       ctypes.c_longlong, _P]),
     "pvd_cnn_apply_fallback": (_I, [_P, _P, _D, _P, ctypes.c_longlong, _P]),
+    "pvd_cnn_input_maxima": (_I, [_P, _P, ctypes.c_longlong, _P, _P]),
     # include/pvd_tissue.h
     "pvd_tissue_map": (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P]),
     # include/pvd_resample.h
@@ -155,6 +156,7 @@ PVD_CNN_EPI_RELU = 0
 PVD_CNN_EPI_RESID_RELU = 1
 PVD_CNN_EPI_LINEAR_F32 = 2
 PVD_CNN_EPI_ACCUM_F32 = 3
+PVD_CNN_MAXIMA_FLOATS = 2 + 2 * 1024
 
 _lib = None
 _lock = threading.Lock()

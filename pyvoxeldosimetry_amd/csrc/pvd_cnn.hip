@@ -643,6 +643,58 @@ __global__ __launch_bounds__(256) void cnn_apply(const float *__restrict__ dp, c
     }
 }
 
+// Maxima of the activity and of the plain dose (the network input's scales) in one pass:
+// per-block maxima into part[2 * block + {0, 1}], then one block reduces them into
+// out[0], out[1].  A maximum is exact, whatever the order.
+constexpr int MAX2_BLOCKS = (PVD_CNN_MAXIMA_FLOATS - 2) / 2;   // partial maxima per call
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void cnn_max2_part(const float *__restrict__ a, const float *__restrict__ b,
+                                                     long long n, float *__restrict__ part) {
+    __shared__ float red[2][4];
+    float ma = -INFINITY, mb = -INFINITY;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        ma = fmaxf(ma, a[v]);
+        mb = fmaxf(mb, b[v]);
+    }
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = ma;
+        red[1][threadIdx.x >> 6] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const float *r = red[threadIdx.x];
+        part[2 * blockIdx.x + threadIdx.x] = fmaxf(fmaxf(r[0], r[1]), fmaxf(r[2], r[3]));
+    }
+}
+
+__global__ __launch_bounds__(256) void cnn_max2_final(const float *__restrict__ part, int nb, float *__restrict__ out) {
+    __shared__ float red[2][4];
+    float ma = -INFINITY, mb = -INFINITY;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        ma = fmaxf(ma, part[2 * i]);
+        mb = fmaxf(mb, part[2 * i + 1]);
+    }
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = ma;
+        red[1][threadIdx.x >> 6] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const float *r = red[threadIdx.x];
+        out[threadIdx.x] = fmaxf(fmaxf(r[0], r[1]), fmaxf(r[2], r[3]));
+    }
+}
+
 // Whole-volume fallback decided on the device from cnn_apply's counts: out = dp when
 // counts[1] > 0 and counts[0] > max_exceed_frac * counts[1]; otherwise every block
 // returns at once.
@@ -901,6 +953,19 @@ int pvd_cnn_apply_correction(const float *dp, const float *r, float bound, const
                        region_frac, apply_frac, out, counts, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch correction");
+}
+
+int pvd_cnn_input_maxima(const float *activity, const float *dose_plain, long long n, float *maxima, void *stream) {
+    if (!activity || !dose_plain || !maxima || n < 1) return pvd_set_error(PVD_EINVAL, "bad argument");
+    long long blocks = (n + 2047) / 2048;
+    if (blocks > MAX2_BLOCKS) blocks = MAX2_BLOCKS;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(cnn_max2_part, dim3((unsigned)blocks), dim3(256), 0, st, activity, dose_plain, n, maxima + 2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return pvd_hip_error(e, "launch input maxima");
+    hipLaunchKernelGGL(cnn_max2_final, dim3(1), dim3(256), 0, st, (const float *)(maxima + 2), (int)blocks, maxima);
+    e = hipGetLastError();
+    return e == hipSuccess ? PVD_OK : pvd_hip_error(e, "launch input maxima");
 }
 
 int pvd_cnn_apply_fallback(const float *dp, const unsigned long long *counts, double max_exceed_frac, float *out,

@@ -164,6 +164,16 @@ class LearnedCorrector:
             self._bufs[key] = torch.empty((n, cols) if cols else tuple(shape), dtype=dtype, device=self.device)
         return self._bufs[key]
 
+    def input_maxima(self, act, dp):
+        """(max activity, max dp) as 1-element device tensors, both in one pass
+        (pvd_cnn_input_maxima); a fresh buffer per call, so earlier results stay valid."""
+        import torch
+        m = torch.empty(nat.PVD_CNN_MAXIMA_FLOATS, dtype=torch.float32, device=self.device)
+        nat.check(self.lib.pvd_cnn_input_maxima(ctypes.c_void_p(act.data_ptr()), ctypes.c_void_p(dp.data_ptr()),
+                                                act.numel(), ctypes.c_void_p(m.data_ptr()), self._stream()),
+                  "pvd_cnn_input_maxima")
+        return m[0:1], m[1:2]
+
     def pack(self, act, rho, dp, scales=None):
         """BF16 network input; returns (x, max dp as a 1-element device tensor).
         The scales are read on the device: no host synchronisation.  ``scales``:
@@ -172,8 +182,7 @@ class LearnedCorrector:
         import torch
         x = self._buf(act.shape, "x", torch.bfloat16, 16)
         if scales is None:
-            amax = act.max().reshape(1)
-            dmax = dp.max().reshape(1)
+            amax, dmax = self.input_maxima(act, dp)
         else:
             amax, dmax = scales
         nat.check(self.lib.pvd_cnn_pack_inputs(ctypes.c_void_p(act.data_ptr()), ctypes.c_void_p(rho.data_ptr()),
@@ -236,8 +245,7 @@ class LearnedCorrector:
         if e < 32:
             raise ValueError("tile_z must be at least 32")
         halo = 6
-        amax = act.max().reshape(1)
-        dmax = dp.max().reshape(1)
+        amax, dmax = self.input_maxima(act, dp)
         r = torch.empty(shape, dtype=torch.float32, device=self.device)
         for s0, s1, a, b in z_slabs(nz, e, halo):
             sl = (slice(None), slice(None), slice(s0, s1))

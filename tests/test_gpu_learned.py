@@ -428,6 +428,29 @@ def test_correction_fallback_rule():
     torch.testing.assert_close(c.dose, dp)
 
 
+def test_input_maxima_equal_torch_max():
+    """pvd_cnn_input_maxima gives max(activity) and max(dose) exactly (a maximum has no
+    rounding): tiny, ragged, all-negative and more-than-one-voxel-per-thread sizes."""
+    import ctypes
+    import torch
+    from pyvoxeldosimetry_amd import _native as nat
+    _say("input maxima")
+    lib = nat.load()
+    g = torch.Generator(device="cuda").manual_seed(11)
+    for n in [1, 255, 70001, 2048 * 1024 + 7, 5_000_000]:
+        a = torch.randn(n, device="cuda", generator=g) * 3.0
+        d = -torch.rand(n, device="cuda", generator=g) - 0.5          # all negative
+        if n > 1:
+            a[n // 3] = 1e6                                           # one spike inside
+        m = torch.full((nat.PVD_CNN_MAXIMA_FLOATS,), float("nan"), device="cuda")
+        nat.check(lib.pvd_cnn_input_maxima(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(d.data_ptr()), n,
+                                           ctypes.c_void_p(m.data_ptr()), None))
+        assert m[0].item() == a.max().item(), n
+        assert m[1].item() == d.max().item(), n
+    with pytest.raises(ValueError):
+        nat.check(lib.pvd_cnn_input_maxima(None, None, 0, None, None))
+
+
 def test_device_fallback_decision_at_its_boundary():
     """pvd_cnn_apply_fallback copies the plain dose exactly when exceed > frac * region
     (the host rule of Correction.fallback), with no host synchronisation."""

@@ -186,7 +186,8 @@ def test_shipped_cnn_weights_are_well_formed():
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
-    """sizeof/offsetof of every ABI struct as gcc lays it out == the ctypes mirror."""
+    """sizeof/offsetof of every ABI struct as gcc lays it out, and the header's constants,
+    == the ctypes mirror."""
     import shutil
     import subprocess
     from pyvoxeldosimetry_amd import _native as nat
@@ -203,6 +204,10 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
         for f in fl:
             lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    consts = ["PVD_CNN_EPI_RELU", "PVD_CNN_EPI_RESID_RELU", "PVD_CNN_EPI_LINEAR_F32", "PVD_CNN_EPI_ACCUM_F32",
+              "PVD_CNN_MAXIMA_FLOATS"]
+    for c in consts:
+        lines.append(f'printf("{c} %d\\n", (int)({c}));')
     lines.append("return 0; }")
     src = tmp_path / "layout.c"
     src.write_text("\n".join(lines))
@@ -214,6 +219,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         assert int(got[name]) == ctypes.sizeof(cls), name
         for f in fl:
             assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, (name, f)
+    for c in consts:
+        assert int(got[c]) == getattr(nat, c), c
 
 
 def test_subregion_box_centres_on_the_maximum_and_stays_inside():
