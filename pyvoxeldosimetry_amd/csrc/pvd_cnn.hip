@@ -233,10 +233,19 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
 // next-halo prefetch and the epilogue are cnn_conv_fwd's.
 constexpr int IN_CHUNKS = 7;
 constexpr int IN_WS = 16 * 2 + 16;                              // bytes per (chunk, cout) row
-constexpr int IN_HALO = HX * HY * HZ;
-constexpr size_t IN_LDS = (size_t)IN_CHUNKS * 32 * IN_WS + (size_t)IN_HALO * 8 + (size_t)(FWD_THREADS / 64) * FWD_STAGE;
+template <int BX, int BY>
+constexpr int in_halo() { return (BX + 2) * (BY + 2) * HZ; }
+template <int BX, int BY>
+constexpr size_t in_lds_bytes() {
+    return (size_t)IN_CHUNKS * 32 * IN_WS + (size_t)in_halo<BX, BY>() * 8 + (size_t)(FWD_THREADS / 64) * FWD_STAGE;
+}
 
+// BX x BY rows per block (2 x 4: the blocks of region lists; 4 x 4 for whole volumes,
+// less halo per output); each of the 8 waves computes BX * BY / 8 rows.
+template <int BX, int BY>
 __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
+    constexpr int IN_HALO = in_halo<BX, BY>(), HYL = BY + 2, NR = BX * BY / 8;
+    static_assert(NR * 8 == BX * BY, "whole rows per wave");
     extern __shared__ uint4 lds_u4[];
     char *lds = reinterpret_cast<char *>(lds_u4);
     char *wl = lds;
@@ -249,22 +258,21 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
         const uint16_t v = tap < 27 ? a.w[((size_t)tap * 32 + co) * 16 + ch] : (uint16_t)0;
         *reinterpret_cast<uint16_t *>(wl + (c * 32 + co) * IN_WS + k * 2) = v;
     }
-    const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
+    const int nbx = (a.n0 + BX - 1) / BX, nby = (a.n1 + BY - 1) / BY, nbz = a.n2 >> 5;
     const int total = a.list ? *a.list_n : nbx * nby * nbz;
     auto block_of = [&](int i) { return a.list ? a.list[i] : i; };
     const int wave = tid >> 6, lane = tid & 63;
-    const int wx = wave / FWD_BY, wy = wave % FWD_BY;
     const int m = lane & 31, kh = lane >> 5;
     const float bco = a.bias ? a.bias[m] : 0.0f;
     constexpr int PER_T = (IN_HALO + FWD_THREADS - 1) / FWD_THREADS;
     uint2 pf[PER_T];
     auto fetch = [&](int b) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * 32;
 #pragma unroll
         for (int u = 0; u < PER_T; ++u) {
             const int vox = tid + u * FWD_THREADS;
-            const int hz = vox % HZ, hy = (vox / HZ) % HY, hx = vox / (HZ * HY);
+            const int hz = vox % HZ, hy = (vox / HZ) % HYL, hx = vox / (HZ * HYL);
             const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
             pf[u] = make_uint2(0u, 0u);
             if (vox < IN_HALO && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2)
@@ -275,7 +283,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
     for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
         const int b = block_of(bi);
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * 32;
         lds_barrier();     // weights visible / previous block finished with the halo
 #pragma unroll
         for (int u = 0; u < PER_T; ++u) {
@@ -283,10 +291,13 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
             if (vox < IN_HALO) *reinterpret_cast<uint2 *>(hl + vox * 8) = pf[u];
         }
         lds_barrier();
+        if (bi + (int)gridDim.x < total) fetch(block_of(bi + gridDim.x));
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+        const int wx = (wave + 8 * r) / BY, wy = (wave + 8 * r) % BY;
         const int ox = x0 + wx, oy = y0 + wy;
         const bool live = ox < a.n0 && oy < a.n1;    // wave-uniform
         const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
-        if (bi + (int)gridDim.x < total) fetch(block_of(bi + gridDim.x));
         if (!live) continue;
         f32x16 acc0 = {}, acc1 = {};
 #pragma unroll
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
                 const int tap = 4 * c + 2 * kh + j;
                 const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
                 h[j] = tap < 27 ? *reinterpret_cast<const uint2 *>(
-                                      hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * 8)
+                                      hl + (((wx + dx + 1) * HYL + (wy + dy + 1)) * HZ + (m + dz + 1)) * 8)
                                 : make_uint2(0u, 0u);
             }
             const bf16x8 fa = __builtin_bit_cast(bf16x8, make_uint4(h[0].x, h[0].y, h[1].x, h[1].y));
@@ -315,6 +326,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
         uint4 *dst = reinterpret_cast<uint4 *>(a.y + vrow * 32);
         dst[lane] = src[lane];
         dst[lane + 64] = src[lane + 64];
+        }
     }
 }
 
@@ -328,11 +340,17 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_in(ConvArgs a) {
 // 27-term sum per output voxel from LDS, in tap order.  Blocks and the region list are
 // those of cnn_conv_fwd (2 x 4 rows of 32 voxels along z).
 constexpr int OUT_YS = 28;                                     // floats per halo voxel of Y
-constexpr int OUT_HALO = HX * HY * HZ;                         // 816 voxels
-constexpr int OUT_GROUPS = (OUT_HALO + 31) / 32;               // 26 row blocks of 32
-constexpr size_t OUT_LDS = (size_t)OUT_HALO * OUT_YS * 4;
+template <int BX, int BY>
+constexpr size_t out_lds_bytes() { return (size_t)(BX + 2) * (BY + 2) * HZ * OUT_YS * 4; }
 
+// BX x BY rows per block: 2 x 4 (816 halo voxels, 26 row blocks of 32) for region lists,
+// 4 x 4 (1224, 39) for whole volumes; one thread per output voxel in the sums.
+template <int BX, int BY>
 __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_out(ConvArgs a) {
+    constexpr int HYL = BY + 2;
+    constexpr int OUT_HALO = (BX + 2) * HYL * HZ;
+    constexpr int OUT_GROUPS = (OUT_HALO + 31) / 32;
+    static_assert(BX * BY * 32 <= FWD_THREADS, "one thread per output voxel");
     extern __shared__ uint4 lds_u4[];
     float *ys = reinterpret_cast<float *>(lds_u4);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -348,18 +366,18 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_out(ConvArgs a) {
         bw[cc] = __builtin_bit_cast(bf16x8, v);
     }
     const float b0 = a.bias ? a.bias[0] : 0.0f;
-    const int nbx = (a.n0 + FWD_BX - 1) / FWD_BX, nby = (a.n1 + FWD_BY - 1) / FWD_BY, nbz = a.n2 >> 5;
+    const int nbx = (a.n0 + BX - 1) / BX, nby = (a.n1 + BY - 1) / BY, nbz = a.n2 >> 5;
     const int total = a.list ? *a.list_n : nbx * nby * nbz;
     for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
         const int b = a.list ? a.list[bi] : bi;
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * 32;
         // A fragments of this wave's row blocks, all loads issued before the MFMAs
         uint4 av[GPW][2];
 #pragma unroll
         for (int j = 0; j < GPW; ++j) {
             const int u = (wave + j * NW) * 32 + m;
-            const int hz = u % HZ, hy = (u / HZ) % HY, hx = u / (HZ * HY);
+            const int hz = u % HZ, hy = (u / HZ) % HYL, hx = u / (HZ * HYL);
             const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
             const bool in = u < OUT_HALO && gx >= 0 && gx < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2;
             const uint4 *src = reinterpret_cast<const uint4 *>(
@@ -387,13 +405,13 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_out(ConvArgs a) {
             }
         }
         lds_barrier();
-        if (tid < FWD_BX * FWD_BY * 32) {
-            const int ox = tid >> 7, oy = (tid >> 5) & 3, z = tid & 31;
+        if (tid < BX * BY * 32) {
+            const int ox = tid / (BY * 32), oy = (tid >> 5) % BY, z = tid & 31;
             if (x0 + ox < a.n0 && y0 + oy < a.n1) {
                 float sum = 0.0f;
                 for (int t = 0; t < 27; ++t) {
                     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-                    sum += ys[(((ox + dx + 1) * HY + (oy + dy + 1)) * HZ + (z + dz + 1)) * OUT_YS + t];
+                    sum += ys[(((ox + dx + 1) * HYL + (oy + dy + 1)) * HZ + (z + dz + 1)) * OUT_YS + t];
                 }
                 a.y32[((long long)(x0 + ox) * a.n1 + (y0 + oy)) * a.n2 + z0 + z] = sum + b0;
             }
@@ -766,28 +784,46 @@ hipError_t launch_fwd(const ConvArgs &a, int blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int BX, int BY>
 hipError_t launch_in(const ConvArgs &a, int blocks, hipStream_t st) {
     static bool attr = false;
+    constexpr size_t lds = in_lds_bytes<BX, BY>();
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_in, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)IN_LDS);
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_in<BX, BY>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(cnn_conv_in, dim3(blocks), dim3(FWD_THREADS), IN_LDS, st, a);
+    hipLaunchKernelGGL((cnn_conv_in<BX, BY>), dim3(blocks), dim3(FWD_THREADS), lds, st, a);
     return hipGetLastError();
 }
 
+template <int BX, int BY>
 hipError_t launch_out(const ConvArgs &a, int blocks, hipStream_t st) {
     static bool attr = false;
+    constexpr size_t lds = out_lds_bytes<BX, BY>();
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_out, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)OUT_LDS);
+        hipError_t e = hipFuncSetAttribute((const void *)cnn_conv_out<BX, BY>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(cnn_conv_out, dim3(blocks), dim3(FWD_THREADS), OUT_LDS, st, a);
+    hipLaunchKernelGGL((cnn_conv_out<BX, BY>), dim3(blocks), dim3(FWD_THREADS), lds, st, a);
     return hipGetLastError();
+}
+
+// whole volumes (no region list): 4 x 4-row blocks for the input and output layers
+#ifndef PVD_CNN_IO44
+#define PVD_CNN_IO44 1
+#endif
+template <bool IN>
+hipError_t launch_io(const ConvArgs &a, int grid, bool whole, hipStream_t st) {
+    if (whole && PVD_CNN_IO44) {
+        const long long t = (long long)((a.n0 + 3) / 4) * ((a.n1 + 3) / 4) * (a.n2 >> 5);
+        const int g = (int)(t < num_cus() ? t : num_cus());
+        return IN ? launch_in<4, 4>(a, g, st) : launch_out<4, 4>(a, g, st);
+    }
+    return IN ? launch_in<FWD_BX, FWD_BY>(a, grid, st) : launch_out<FWD_BX, FWD_BY>(a, grid, st);
 }
 
 template <int CIN>
@@ -848,7 +884,7 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     if (cin == 4) {
-        e = launch_in(a, grid, st);
+        e = launch_io<true>(a, grid, blocks == nullptr, st);
     } else if (cin == 16) {
         switch (epi) {
             case 0: e = launch_fwd<16, 0>(a, grid, st); break;
@@ -860,7 +896,7 @@ int pvd_cnn_conv_fwd(const void *x, int cin, const void *w, const float *bias, i
         switch (epi) {
             case 0: e = launch_fwd<32, 0>(a, grid, st); break;
             case 1: e = launch_fwd<32, 1>(a, grid, st); break;
-            case 2: e = launch_out(a, grid, st); break;
+            case 2: e = launch_io<false>(a, grid, blocks == nullptr, st); break;
             default: e = launch_fwd<32, 3>(a, grid, st); break;
         }
     }
