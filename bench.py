@@ -438,20 +438,22 @@ def torso_cnn(dev, acts, ph, rho, args):
     import torch
 
     from pyvoxeldosimetry_amd.core.activity_sampler import trapezoid_weights
-    from pyvoxeldosimetry_amd.learned.pipeline import LearnedCorrector, evaluate_subregion, subregion_box
+    from pyvoxeldosimetry_amd.learned.pipeline import (LearnedCorrector, evaluate_subregion, slab_extent,
+                                                       subregion_box)
     corr = LearnedCorrector.pretrained("Lu177", ph.voxel_mm)
     w = [x * 3600.0 for x in trapezoid_weights(ph.time_points_h)]
     cum = torch.zeros_like(acts[0])
     for wi, a in zip(w, acts):
         cum.add_(a, alpha=wi)
     ms_plain = _timed(lambda: corr.plain(cum, rho), 5, 1)
-    ms_corr = _timed(lambda: corr.correct(cum, rho, tile_z=128), 3, 1)
-    c = corr.correct(cum, rho, tile_z=128)
+    ms_corr = _timed(lambda: corr.correct(cum, rho), 3, 1)
+    c = corr.correct(cum, rho)
+    ext = slab_extent(cum.shape[2], cum.shape[0] * cum.shape[1])
     out = {"ms_per_vol": round(ms_corr, 3), "plain_ms_per_vol": round(ms_plain, 3),
            "ratio_corrected_over_plain": round(ms_corr / ms_plain, 2), "fallback": bool(c.fallback),
            "workload": "torso 512x512x400 @0.98 mm, time-integrated activity of the 6 timepoints, Lu-177 "
-                       "Monte Carlo point kernel %d^3 (linear), 6-layer 3^3 residual CNN (BF16 MFMA) in z slabs of 128 "
-                       "(12-voxel overlaps), bounded correction" % corr.kernel.shape[0]}
+                       "Monte Carlo point kernel %d^3 (linear), 6-layer 3^3 residual CNN (BF16 MFMA) in z slabs of %d "
+                       "(12-voxel overlaps), bounded correction" % (corr.kernel.shape[0], ext)}
     if not args.no_eval:
         sx, sy, sz = cum.shape
         boxes = {"hottest_lesion": subregion_box(cum, 64),

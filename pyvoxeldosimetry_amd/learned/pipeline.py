@@ -272,7 +272,8 @@ class LearnedCorrector:
         dp = self.plain(act, rho, out=self._buf(shape, "dp", act.dtype))
         apply_frac = self.region_frac if region else 0.0
         if r_override is None and (tile_z is not None or shape[2] % 32):
-            r, dmax = self.network_tiled(act, rho, dp, tile_z or 128, region)
+            ext = tile_z or slab_extent(shape[2], shape[0] * shape[1])
+            r, dmax = self.network_tiled(act, rho, dp, ext, region)
         elif r_override is None:
             x, dmax = self.pack(act, rho, dp)
             blocks = self.region_blocks(dp, dmax, apply_frac) if region else None
@@ -294,6 +295,25 @@ class LearnedCorrector:
                                                   ctypes.c_void_p(out.data_ptr()), dp.numel(), self._stream()),
                   "pvd_cnn_apply_fallback")
         return Correction(out, dp, r.reshape(shape), counts, self.max_exceed_frac)
+
+
+def slab_extent(nz: int, plane_voxels: int, halo: int = 6, max_bytes: float = 32 * 2 ** 30) -> int:
+    """The z extent (a multiple of 32, at most nz) whose slabs (`z_slabs`) cover nz with
+    the fewest computed planes, among those whose activations (16 + 5 x 32 BF16
+    channels per voxel) stay within max_bytes; the smaller extent on a tie.  For a
+    512 x 512 x 400 volume: 224 (two slabs, 448 planes; slabs of 128 compute 512)."""
+    best = None
+    for e in range(32, nz // 32 * 32 + 1, 32):
+        if e <= 2 * halo:
+            continue
+        if e > 32 and plane_voxels * e * (16 + 5 * 32) * 2 > max_bytes:
+            break
+        planes = sum(s1 - s0 for s0, s1, _, _ in z_slabs(nz, e, halo))
+        if best is None or planes < best[0]:
+            best = (planes, e)
+    if best is None:
+        raise ValueError(f"the network needs at least 32 voxels along z (got {nz})")
+    return best[1]
 
 
 def z_slabs(nz: int, e: int, halo: int = 6):

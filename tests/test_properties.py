@@ -174,3 +174,21 @@ def test_network_slabs_tile_the_axis_and_keep_the_receptive_field(nz, k):
         assert s1 - s0 == e and 0 <= s0 <= a < b <= s1 <= nz
         assert a == 0 or a - s0 >= 6
         assert b == nz or s1 - b >= 6
+
+
+@settings(max_examples=200, deadline=None, derandomize=True)
+@given(nz=st.integers(32, 1200), plane=st.sampled_from([1, 64 * 64, 512 * 512]))
+def test_automatic_slab_extent_computes_the_fewest_planes(nz, plane):
+    """slab_extent: a multiple of 32 within [32, nz], within the memory cap (or 32), and
+    no allowed extent computes fewer planes (brute force over every extent)."""
+    from pyvoxeldosimetry_amd.learned.pipeline import slab_extent, z_slabs
+    cap = 8 * 2 ** 30
+    e = slab_extent(nz, plane, max_bytes=cap)
+    assert e % 32 == 0 and 32 <= e <= nz
+    per = plane * (16 + 5 * 32) * 2
+    assert e == 32 or per * e <= cap
+    planes = lambda x: sum(s1 - s0 for s0, s1, _, _ in z_slabs(nz, x))  # noqa: E731
+    allowed = [x for x in range(32, nz // 32 * 32 + 1, 32) if x == 32 or per * x <= cap]
+    assert planes(e) == min(planes(x) for x in allowed)
+    if nz == 400 and plane == 512 * 512:
+        assert e == 224 and planes(e) == 448
