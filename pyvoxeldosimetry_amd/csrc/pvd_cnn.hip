@@ -112,21 +112,28 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
                 pf[u] = reinterpret_cast<const uint4 *>(a.x + (((long long)gx * a.n1 + gy) * a.n2 + gz) * CIN)[c8];
         }
     };
-    if ((int)blockIdx.x < total) fetch(block_of(blockIdx.x));
-    for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
-        const int b = block_of(bi);
-        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
-        // LDS-only barriers: the previous block's output stores stay in flight (a
-        // full __syncthreads would wait for them); the prefetched halo registers are
-        // waited for where they are written
-        lds_barrier();     // weights visible / previous block finished with the halo
+    auto put_halo = [&]() {
 #pragma unroll
         for (int u = 0; u < PER_T; ++u) {
             const int i = tid + u * FWD_THREADS;
             if (i < HALO) *reinterpret_cast<uint4 *>(hl + (i / C8) * VS + (i % C8) * 16) = pf[u];
         }
-        lds_barrier();
+    };
+    // Per block: MFMA loop, the skip values out of the halo, barrier, the next block's
+    // halo into LDS, then the epilogue, whose output stores overlap the next block's
+    // MFMAs.  (The wait for the prefetched halo registers counts every earlier vector
+    // memory operation; the latest stores are then a block old.)
+    if ((int)blockIdx.x < total) {
+        fetch(block_of(blockIdx.x));
+        put_halo();
+    }
+    for (int bi = blockIdx.x; bi < total; bi += gridDim.x) {
+        const int b = block_of(bi);
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * FWD_BX, y0 = by * FWD_BY, z0 = bz * 32;
+        // LDS-only barriers: the previous block's output stores stay in flight (a
+        // full __syncthreads would wait for them)
+        lds_barrier();     // weights and this block's halo visible
         const int ox = x0 + wx, oy = y0 + wy;
         const bool live = ox < a.n0 && oy < a.n1;    // wave-uniform
         const long long vrow = ((long long)ox * a.n1 + oy) * a.n2 + z0;
@@ -141,41 +148,47 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
             for (int i = 0; i < 16; ++i) rs[i] = a.resid[(vrow + mfma_row(i, lane)) * 32 + m];
         }
         if (bi + (int)gridDim.x < total) fetch(block_of(bi + gridDim.x));
-        if (!live) continue;
-        // two independent accumulation chains (by tap parity), so consecutive MFMAs
-        // of a wave do not wait for each other's results
-        // The fragments of tap t + 1 are read before the MFMAs of tap t (two register
-        // sets), so each LDS round trip overlaps the previous tap's MFMAs; the MFMA
-        // sequence into each accumulator is the unpipelined loop's.
-        f32x16 acc0 = {}, acc1 = {};
-        constexpr int KC = CIN / 16;
-        bf16x8 fa[2][KC], fb[2][KC];
-        auto read_tap = [&](int tap, bf16x8 *ta, bf16x8 *tb) {
-            const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-            const char *hrow = hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * VS + kh * 16;
-            const char *wrow = wl + (tap * 32 + m) * WS + kh * 16;
+        // one accumulation chain in tap order; the fragments of tap t + 1 are read
+        // before the MFMAs of tap t (two register sets), so each LDS round trip
+        // overlaps the previous tap's MFMAs
+        f32x16 acc = {};
+        if (live) {
+            constexpr int KC = CIN / 16;
+            bf16x8 fa[2][KC], fb[2][KC];
+            auto read_tap = [&](int tap, bf16x8 *ta, bf16x8 *tb) {
+                const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+                const char *hrow = hl + (((wx + dx + 1) * HY + (wy + dy + 1)) * HZ + (m + dz + 1)) * VS + kh * 16;
+                const char *wrow = wl + (tap * 32 + m) * WS + kh * 16;
 #pragma unroll
-            for (int cc = 0; cc < KC; ++cc) {
-                ta[cc] = lds_frag(hrow + cc * 32);
-                tb[cc] = lds_frag(wrow + cc * 32);
+                for (int cc = 0; cc < KC; ++cc) {
+                    ta[cc] = lds_frag(hrow + cc * 32);
+                    tb[cc] = lds_frag(wrow + cc * 32);
+                }
+            };
+            read_tap(0, fa[0], fb[0]);
+#pragma unroll
+            for (int tap = 0; tap < 27; ++tap) {
+                const int cur = tap & 1;
+                if (tap + 1 < 27) read_tap(tap + 1, fa[cur ^ 1], fb[cur ^ 1]);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int cc = 0; cc < KC; ++cc)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[cur][cc], fb[cur][cc], acc, 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
             }
-        };
-        read_tap(0, fa[0], fb[0]);
+            if (resid_lds) {   // halo voxel (wx + 1, wy + 1, z + 1), channel m
 #pragma unroll
-        for (int tap = 0; tap < 27; ++tap) {
-            const int cur = tap & 1;
-            if (tap + 1 < 27) read_tap(tap + 1, fa[cur ^ 1], fb[cur ^ 1]);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int cc = 0; cc < KC; ++cc) {
-                if (((tap + cc) & 1) == 0)
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[cur][cc], fb[cur][cc], acc0, 0, 0, 0);
-                else
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[cur][cc], fb[cur][cc], acc1, 0, 0, 0);
+                for (int i = 0; i < 16; ++i)
+                    rs[i] = *reinterpret_cast<const uint16_t *>(
+                        hl + (((wx + 1) * HY + (wy + 1)) * HZ + mfma_row(i, lane) + 1) * VS + m * 2);
             }
-            __builtin_amdgcn_sched_barrier(0);
         }
-        const f32x16 acc = acc0 + acc1;
+        lds_barrier();     // every wave finished with this block's halo
+        // unconditional, so that no path leaves the prefetch loads outstanding (they
+        // would be waited for, with the epilogue's stores, at the next fetch); after
+        // the last block it rewrites values that nothing reads
+        put_halo();
+        if (!live) continue;
         const int co = m;
         if (EPI == PVD_CNN_EPI_RELU || EPI == PVD_CNN_EPI_RESID_RELU) {
             // The wave's output tile (voxels z0..z0+31 of one column, 32 channels) is
@@ -198,11 +211,7 @@ __global__ __launch_bounds__(FWD_THREADS) void cnn_conv_fwd(ConvArgs a) {
                 if (EPI == PVD_CNN_EPI_RELU) {
                     h = f2bf(av);
                 } else {
-                    uint16_t r = rs[i];
-                    if (resid_lds)   // halo voxel (wx + 1, wy + 1, z + 1), channel co
-                        r = *reinterpret_cast<const uint16_t *>(
-                            hl + (((wx + 1) * HY + (wy + 1)) * HZ + mfma_row(i, lane) + 1) * VS + co * 2);
-                    h = f2bf(bf2f(r) + av);
+                    h = f2bf(bf2f(rs[i]) + av);
                 }
                 sy[mfma_row(i, lane) * 32 + co] = h;
             }
