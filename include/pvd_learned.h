@@ -129,7 +129,8 @@ int pvd_cnn_pack_inputs(const float *activity, const float *density, const float
                         const float *dose_max, void *x, long long n_voxels, void *stream);
 
 /* The network input's scales in one pass: maxima[0] = max(activity), maxima[1] =
- * max(dose_plain), for pvd_cnn_pack_inputs' act_max = maxima and dose_max = maxima + 1.
+ * max(dose_plain), for pvd_cnn_pack_inputs' act_max = maxima and dose_max = maxima + 1
+ * (NaN values are skipped, as fmaxf does; -inf if every value is NaN).
  * maxima holds PVD_CNN_MAXIMA_FLOATS floats; those past the first two are scratch. */
 #define PVD_CNN_MAXIMA_FLOATS (2 + 2 * 1024)
 int pvd_cnn_input_maxima(const float *activity, const float *dose_plain, long long n_voxels, float *maxima,
